@@ -1,0 +1,973 @@
+// Device context of the per-step path: owns the device copies of every
+// shell's per-triangle and per-node streams, the two position/velocity
+// buffers, the chunk tables of the ragged batch, the HIP stream and the
+// captured step graphs, and implements the C-ABI entry points that mirror the
+// reference's per-step calls.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "ms_common.h"
+
+using namespace ms;
+
+#define MS_VERSION "0.1.0"
+
+struct ms_ctx {
+    std::string err;
+    int precision = MS_FP64;
+    int num_shells = 0;
+    int T = 0, N = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+
+    std::vector<ShellDev> hs;  // host mirror of the per-shell records
+    ShellDev* d_shells = nullptr;
+    ClockDev* d_clock = nullptr;
+    long long steps = 0;       // host mirror of the step clock
+
+    // device streams
+    int* pid = nullptr;
+    double *G = nullptr, *H = nullptr, *ainfo = nullptr, *binfo = nullptr, *a0 = nullptr, *b0 = nullptr;
+    double *h = nullptr, *mu = nullptr, *area = nullptr;
+    float *G32 = nullptr, *H32 = nullptr, *hmuA32 = nullptr;
+    double* mass = nullptr;
+    unsigned char* clamp = nullptr;
+    long long* csr_off = nullptr;
+    int* csr_ent = nullptr;
+    Buffers b{};
+    double* scratch3 = nullptr;  // 9T (geometry outputs) / 2T densities
+    double* red_out = nullptr;   // per-shell reduction outputs (3 per shell)
+
+    // launch shapes
+    ms_tuning tune{};
+    Chunk* d_tri_chunks = nullptr;
+    Chunk* d_node_chunks = nullptr;
+    int n_tri_chunks = 0, n_node_chunks = 0;
+    size_t part_len = 0;
+
+    // graphs
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    int graph_precision = -1;
+
+    // timing
+    bool timing = false;
+    double last_ms[3] = {0, 0, 0};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+    int64_t dev_bytes = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(ms_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    set_last_error(msg);
+    return code;
+}
+
+#define HIPC(ctx, expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return fail((ctx), MS_ERR_HIP, fmt("%s: %s", #expr, hipGetErrorString(e_))); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(ms_ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e == hipSuccess) {
+        c->allocs.push_back(*p);
+        c->dev_bytes += (int64_t)(n * sizeof(T));
+    }
+    return e;
+}
+
+int build_chunks(ms_ctx* c) {
+    const int tb = c->tune.tri_block, nb = c->tune.node_block;
+    std::vector<Chunk> tc, nc;
+    for (int s = 0; s < c->num_shells; ++s) {
+        ShellDev& S = c->hs[s];
+        S.tri_chunk_begin = (int)tc.size();
+        for (int t = S.tri_begin; t < S.tri_begin + S.num_tris; t += tb)
+            tc.push_back(make_int4(t, std::min(t + tb, S.tri_begin + S.num_tris), s, 0));
+        S.tri_chunk_count = (int)tc.size() - S.tri_chunk_begin;
+        S.node_chunk_begin = (int)nc.size();
+        for (int n = S.node_begin; n < S.node_begin + S.num_nodes; n += nb)
+            nc.push_back(make_int4(n, std::min(n + nb, S.node_begin + S.num_nodes), s, 0));
+        S.node_chunk_count = (int)nc.size() - S.node_chunk_begin;
+    }
+    if (c->d_tri_chunks) hipFree(c->d_tri_chunks);
+    if (c->d_node_chunks) hipFree(c->d_node_chunks);
+    if (c->b.part) hipFree(c->b.part);
+    if (c->b.part_i) hipFree(c->b.part_i);
+    c->d_tri_chunks = c->d_node_chunks = nullptr;
+    c->b.part = nullptr;
+    c->b.part_i = nullptr;
+    HIPC(c, hipMalloc((void**)&c->d_tri_chunks, tc.size() * sizeof(Chunk)));
+    HIPC(c, hipMalloc((void**)&c->d_node_chunks, nc.size() * sizeof(Chunk)));
+    HIPC(c, hipMemcpy(c->d_tri_chunks, tc.data(), tc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->d_node_chunks, nc.data(), nc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    c->n_tri_chunks = (int)tc.size();
+    c->n_node_chunks = (int)nc.size();
+    c->part_len = 4 * (size_t)std::max(tc.size(), nc.size());
+    HIPC(c, hipMalloc((void**)&c->b.part, c->part_len * sizeof(double)));
+    HIPC(c, hipMalloc((void**)&c->b.part_i, c->part_len * sizeof(int)));
+    return MS_OK;
+}
+
+// Host mirror <-> device records.
+int pull_shells(ms_ctx* c) {
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipMemcpy(c->hs.data(), c->d_shells, c->hs.size() * sizeof(ShellDev), hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+int push_shells(ms_ctx* c) {
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipMemcpy(c->d_shells, c->hs.data(), c->hs.size() * sizeof(ShellDev), hipMemcpyHostToDevice));
+    return MS_OK;
+}
+
+TriData tri_data(const ms_ctx* c) {
+    TriData td{};
+    td.T = c->T;
+    td.N = c->N;
+    td.pid = c->pid;
+    td.G = c->G;
+    td.H = c->H;
+    td.G32 = c->G32;
+    td.H32 = c->H32;
+    td.ainfo = c->ainfo;
+    td.binfo = c->binfo;
+    td.a0 = c->a0;
+    td.b0 = c->b0;
+    td.h = c->h;
+    td.mu = c->mu;
+    td.area = c->area;
+    td.hmuA32 = c->hmuA32;
+    return td;
+}
+
+NodeData node_data(const ms_ctx* c) {
+    NodeData nd{};
+    nd.N = c->N;
+    nd.mass = c->mass;
+    nd.clamp = c->clamp;
+    nd.csr_off = c->csr_off;
+    nd.csr_ent = c->csr_ent;
+    return nd;
+}
+
+int grid_for(const ms_ctx* c, int nchunks, int per_cu) {
+    if (per_cu <= 0) return std::max(1, nchunks);
+    int g = c->num_cus * per_cu;
+    g = ((g + 7) / 8) * 8;
+    return std::max(1, std::min(g, nchunks));
+}
+Launch tri_launch(const ms_ctx* c) {
+    return Launch{grid_for(c, c->n_tri_chunks, c->tune.tri_grid_per_cu), c->tune.tri_block, c->stream};
+}
+Launch node_launch(const ms_ctx* c) {
+    return Launch{grid_for(c, c->n_node_chunks, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
+}
+
+void params_to_dev(const ms_shell_params& p, ShellDev& S) {
+    S.nu = p.poisson_ratio;
+    S.gamma = p.bend_stiffening_scale_factor;
+    S.dt = p.timestep;
+    S.eta = p.damping_factor;
+    S.damp_den = p.ref_density * p.min_ref_thickness * p.min_ref_thickness;
+    S.lim = 50000.0 * p.char_force_scale;
+    S.fchar = p.char_force_scale;
+    S.cchar = p.char_speed_scale;
+    S.gd = p.gradient_descent ? 1 : 0;
+    S.protocol = p.dial_protocol == MS_DIAL_LCE ? 3 : 1;
+    S.from_ansatz = p.dial_from_ansatz ? 1 : 0;
+}
+
+void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v) {
+    const bool mixed = c->precision == MS_MIXED;
+    TriData td = tri_data(c);
+    NodeData nd = node_data(c);
+    launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, true);
+    StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
+    launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
+                     c->d_clock, mixed, fl);
+}
+
+void destroy_graph(ms_ctx* c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_steps = 0;
+}
+
+int ensure_graph(ms_ctx* c) {
+    const int gs = std::max(1, c->tune.graph_steps);
+    if (c->graph_exec && c->graph_steps == gs && c->graph_precision == c->precision) return MS_OK;
+    destroy_graph(c);
+    hipGraph_t g;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < gs; ++i) step_once(c, false, 0, 0.0, 0.0);
+    HIPC(c, hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, MS_ERR_HIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e)));
+    c->graph_steps = gs;
+    c->graph_precision = c->precision;
+    return MS_OK;
+}
+
+int shell_ok(ms_ctx* c, int32_t s) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    if (s < 0 || s >= c->num_shells) return fail(c, MS_ERR_ARG, fmt("shell %d out of range", s));
+    return MS_OK;
+}
+
+}  // namespace
+
+extern "C" const char* ms_version(void) { return MS_VERSION; }
+
+extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_t precision, ms_ctx** out) {
+    if (!shells || num_shells <= 0 || !out || (precision != MS_FP64 && precision != MS_MIXED))
+        return fail(nullptr, MS_ERR_ARG, "ms_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, MS_ERR_NO_DEVICE, "ms_create: no HIP device visible");
+    // validate and size
+    long long Ttot = 0, Ntot = 0;
+    int acols = 3;
+    bool any_lce = false, any_a0 = false, any_b0 = false;
+    for (int s = 0; s < num_shells; ++s) {
+        const ms_shell_desc& d = shells[s];
+        if (d.num_nodes <= 0 || d.num_tris <= 0 || !d.patch_ids || !d.G || !d.H || !d.ref_areas || !d.thickness ||
+            !d.shear_modulus || !d.abar_info || !d.node_masses)
+            return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d has a missing array", s));
+        const bool lce = d.params.dial_protocol == MS_DIAL_LCE;
+        if (!lce && (d.abar_info_cols != 3 || !d.bbar_info || d.bbar_info_cols != 3))
+            return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d: blend protocol needs 3-column abar/bbar info", s));
+        if (lce && d.abar_info_cols < 3)
+            return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d: LCE protocol needs 3-column abar info", s));
+        if (d.params.dial_protocol != MS_DIAL_LCE && d.params.dial_protocol != MS_DIAL_BLEND)
+            return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d: unknown dial protocol", s));
+        any_lce |= lce;
+        any_a0 |= (d.a0 != nullptr);
+        any_b0 |= (d.b0 != nullptr);
+        for (long long i = 0; i < 6LL * d.num_tris; ++i)
+            if (d.patch_ids[i] < 0 || d.patch_ids[i] >= d.num_nodes)
+                return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d: patch id out of range", s));
+        Ttot += d.num_tris;
+        Ntot += d.num_nodes;
+    }
+    if (Ttot > INT_MAX / 18 || Ntot > INT_MAX / 3)
+        return fail(nullptr, MS_ERR_ARG, "ms_create: batch too large for 32-bit stream indexing");
+    if (any_lce) acols = 5;
+    ms_ctx* c = new ms_ctx();
+    c->precision = precision;
+    c->num_shells = num_shells;
+    c->T = (int)Ttot;
+    c->N = (int)Ntot;
+    c->tune.tri_block = 256;
+    c->tune.node_block = 256;
+    c->tune.tri_grid_per_cu = 0;
+    c->tune.node_grid_per_cu = 0;
+    c->tune.graph_steps = 16;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    auto bail = [&](int code) {
+        ms_destroy(c);
+        return code;
+    };
+    // A blocking stream: the context's synchronous hipMemcpy/hipMemset calls
+    // (legacy null stream) stay ordered with its kernels.
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess) {
+        set_last_error("ms_create: hipStreamCreate failed");
+        return bail(MS_ERR_HIP);
+    }
+    const int T = c->T, N = c->N;
+    // host staging in SoA
+    std::vector<int> pid(6 * (size_t)T);
+    std::vector<double> G(6 * (size_t)T), H(18 * (size_t)T), ai((size_t)acols * T, 0.0), bi(3 * (size_t)T, 0.0);
+    std::vector<double> a0(any_a0 ? 3 * (size_t)T : 0), b0(any_b0 ? 3 * (size_t)T : 0);
+    std::vector<double> hh(T), mm(T), ar(T), mass(N);
+    std::vector<unsigned char> cl(N, 0);
+    c->hs.assign(num_shells, ShellDev{});
+    int tb = 0, nb = 0;
+    for (int s = 0; s < num_shells; ++s) {
+        const ms_shell_desc& d = shells[s];
+        ShellDev& S = c->hs[s];
+        params_to_dev(d.params, S);
+        S.tri_begin = tb;
+        S.num_tris = d.num_tris;
+        S.node_begin = nb;
+        S.num_nodes = d.num_nodes;
+        S.dial_last_eq = 1.0;
+        S.dial_spacing = 0.0;
+        S.dial_K = 1.0;
+        S.halt_step = -1;
+        S.fail_step = -1;
+        S.has_a0 = d.a0 ? 1 : 0;
+        S.has_b0 = d.b0 ? 1 : 0;
+        const bool lce = S.protocol == 3;
+        for (int t = 0; t < d.num_tris; ++t) {
+            const size_t g = (size_t)tb + t;
+            for (int k = 0; k < 6; ++k) pid[k * (size_t)T + g] = d.patch_ids[6 * (size_t)t + k] + nb;
+            for (int k = 0; k < 6; ++k) G[k * (size_t)T + g] = d.G[6 * (size_t)t + k];
+            for (int k = 0; k < 18; ++k) H[k * (size_t)T + g] = d.H[18 * (size_t)t + k];
+            if (lce) {
+                const double* r = d.abar_info + (size_t)t * d.abar_info_cols;
+                const double cth = std::cos(r[0]), sth = std::sin(r[0]);
+                ai[0 * (size_t)T + g] = cth * cth;
+                ai[1 * (size_t)T + g] = cth * sth;
+                ai[2 * (size_t)T + g] = sth * sth;
+                ai[3 * (size_t)T + g] = r[1];
+                ai[4 * (size_t)T + g] = -2.0 * r[2];
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    ai[k * (size_t)T + g] = d.abar_info[(size_t)t * 3 + k];
+                    bi[k * (size_t)T + g] = d.bbar_info[(size_t)t * 3 + k];
+                }
+            }
+            for (int k = 0; k < 3; ++k) {
+                if (any_a0) a0[k * (size_t)T + g] = d.a0 ? d.a0[(size_t)k * d.num_tris + t] : (k == 1 ? 0.0 : 1.0);
+                if (any_b0) b0[k * (size_t)T + g] = d.b0 ? d.b0[(size_t)k * d.num_tris + t] : 0.0;
+            }
+            hh[g] = d.thickness[t];
+            mm[g] = d.shear_modulus[t];
+            ar[g] = d.ref_areas[t];
+        }
+        for (int n = 0; n < d.num_nodes; ++n) {
+            mass[(size_t)nb + n] = d.node_masses[n];
+            cl[(size_t)nb + n] = d.constrained ? (d.constrained[n] ? 1 : 0) : 0;
+        }
+        tb += d.num_tris;
+        nb += d.num_nodes;
+    }
+    // patch-incidence CSR: entries t*6+slot grouped by node, ascending
+    std::vector<long long> off((size_t)N + 1, 0);
+    for (size_t i = 0; i < 6 * (size_t)T; ++i) off[(size_t)pid[i] + 1]++;
+    for (int n = 0; n < N; ++n) off[n + 1] += off[n];
+    std::vector<int> ent(6 * (size_t)T);
+    {
+        std::vector<long long> cur(off.begin(), off.end() - 1);
+        for (int t = 0; t < T; ++t)
+            for (int k = 0; k < 6; ++k) {
+                const int n = pid[k * (size_t)T + t];
+                ent[(size_t)cur[n]++] = t * 6 + k;
+            }
+    }
+#define UP(dst, vec)                                                                                      \
+    do {                                                                                                  \
+        if (!(vec).empty()) {                                                                             \
+            if (dalloc(c, &(dst), (vec).size()) != hipSuccess ||                                          \
+                hipMemcpy((dst), (vec).data(), (vec).size() * sizeof((vec)[0]), hipMemcpyHostToDevice) != \
+                    hipSuccess) {                                                                         \
+                set_last_error("ms_create: device allocation/upload failed for " #dst);                   \
+                return bail(MS_ERR_HIP);                                                                  \
+            }                                                                                             \
+        }                                                                                                 \
+    } while (0)
+    UP(c->pid, pid);
+    UP(c->G, G);
+    UP(c->H, H);
+    UP(c->ainfo, ai);
+    UP(c->binfo, bi);
+    UP(c->a0, a0);
+    UP(c->b0, b0);
+    UP(c->h, hh);
+    UP(c->mu, mm);
+    UP(c->area, ar);
+    UP(c->mass, mass);
+    UP(c->clamp, cl);
+    UP(c->csr_off, off);
+    UP(c->csr_ent, ent);
+    {
+        std::vector<float> g32(G.begin(), G.end()), h32(H.begin(), H.end()), hma(3 * (size_t)T);
+        for (int t = 0; t < T; ++t) {
+            hma[t] = (float)hh[t];
+            hma[T + (size_t)t] = (float)mm[t];
+            hma[2 * (size_t)T + t] = (float)ar[t];
+        }
+        UP(c->G32, g32);
+        UP(c->H32, h32);
+        UP(c->hmuA32, hma);
+    }
+#undef UP
+    bool ok = true;
+    for (int i = 0; i < 2; ++i) {
+        ok &= dalloc(c, &c->b.x[i], 3 * (size_t)N) == hipSuccess;
+        ok &= dalloc(c, &c->b.v[i], 3 * (size_t)N) == hipSuccess;
+    }
+    ok &= dalloc(c, &c->b.F, 3 * (size_t)N) == hipSuccess;
+    ok &= dalloc(c, &c->b.contrib, 18 * (size_t)T) == hipSuccess;
+    ok &= dalloc(c, &c->b.contrib32, 18 * (size_t)T) == hipSuccess;
+    ok &= dalloc(c, &c->b.bad_dof, (size_t)num_shells) == hipSuccess;
+    ok &= dalloc(c, &c->red_out, 3 * (size_t)num_shells) == hipSuccess;
+    ok &= dalloc(c, &c->d_shells, (size_t)num_shells) == hipSuccess;
+    ok &= dalloc(c, &c->d_clock, 1) == hipSuccess;
+    if (!ok) {
+        set_last_error("ms_create: device allocation failed");
+        return bail(MS_ERR_HIP);
+    }
+    for (int i = 0; i < 2; ++i) {
+        hipMemset(c->b.x[i], 0, 3 * sizeof(double) * (size_t)N);
+        hipMemset(c->b.v[i], 0, 3 * sizeof(double) * (size_t)N);
+    }
+    hipMemset(c->b.F, 0, 3 * sizeof(double) * (size_t)N);
+    hipMemset(c->d_clock, 0, sizeof(ClockDev));
+    if (build_chunks(c) != MS_OK) {
+        set_last_error(c->err);
+        return bail(MS_ERR_HIP);
+    }
+    if (push_shells(c) != MS_OK) {
+        set_last_error(c->err);
+        return bail(MS_ERR_HIP);
+    }
+    for (auto& e : c->ev) hipEventCreate(&e);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_last_error("ms_create: device error after upload");
+        return bail(MS_ERR_HIP);
+    }
+    *out = c;
+    return MS_OK;
+}
+
+extern "C" void ms_destroy(ms_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    destroy_graph(c);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->d_tri_chunks) hipFree(c->d_tri_chunks);
+    if (c->d_node_chunks) hipFree(c->d_node_chunks);
+    if (c->b.part) hipFree(c->b.part);
+    if (c->b.part_i) hipFree(c->b.part_i);
+    if (c->scratch3) hipFree(c->scratch3);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* ms_ctx_last_error(const ms_ctx* c) { return c ? c->err.c_str() : "null context"; }
+extern "C" int32_t ms_num_shells(const ms_ctx* c) { return c ? c->num_shells : 0; }
+extern "C" int64_t ms_steps_taken(const ms_ctx* c) { return c ? c->steps : 0; }
+extern "C" int64_t ms_device_bytes(const ms_ctx* c) { return c ? c->dev_bytes : 0; }
+
+extern "C" int ms_set_state(ms_ctx* c, int32_t s, const double* dofs, const double* vel) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!dofs) return fail(c, MS_ERR_ARG, "ms_set_state: dofs is null");
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    const int par = S.par;
+    for (int comp = 0; comp < 3; ++comp) {
+        HIPC(c, hipMemcpy(c->b.x[par] + (size_t)comp * c->N + S.node_begin, dofs + (size_t)comp * S.num_nodes,
+                          sizeof(double) * S.num_nodes, hipMemcpyHostToDevice));
+        if (vel)
+            HIPC(c, hipMemcpy(c->b.v[par] + (size_t)comp * c->N + S.node_begin, vel + (size_t)comp * S.num_nodes,
+                              sizeof(double) * S.num_nodes, hipMemcpyHostToDevice));
+        else
+            HIPC(c, hipMemset(c->b.v[par] + (size_t)comp * c->N + S.node_begin, 0, sizeof(double) * S.num_nodes));
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_get_state(ms_ctx* c, int32_t s, double* dofs, double* vel) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = pull_shells(c)) return r;
+    const ShellDev& S = c->hs[s];
+    for (int comp = 0; comp < 3; ++comp) {
+        if (dofs)
+            HIPC(c, hipMemcpy(dofs + (size_t)comp * S.num_nodes, c->b.x[S.par] + (size_t)comp * c->N + S.node_begin,
+                              sizeof(double) * S.num_nodes, hipMemcpyDeviceToHost));
+        if (vel)
+            HIPC(c, hipMemcpy(vel + (size_t)comp * S.num_nodes, c->b.v[S.par] + (size_t)comp * c->N + S.node_begin,
+                              sizeof(double) * S.num_nodes, hipMemcpyDeviceToHost));
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_get_forces(ms_ctx* c, int32_t s, double* f) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!f) return fail(c, MS_ERR_ARG, "ms_get_forces: null output");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    for (int comp = 0; comp < 3; ++comp)
+        HIPC(c, hipMemcpy(f + (size_t)comp * S.num_nodes, c->b.F + (size_t)comp * c->N + S.node_begin,
+                          sizeof(double) * S.num_nodes, hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+
+extern "C" int ms_set_forces(ms_ctx* c, int32_t s, const double* f) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!f) return fail(c, MS_ERR_ARG, "ms_set_forces: null input");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    for (int comp = 0; comp < 3; ++comp)
+        HIPC(c, hipMemcpy(c->b.F + (size_t)comp * c->N + S.node_begin, f + (size_t)comp * S.num_nodes,
+                          sizeof(double) * S.num_nodes, hipMemcpyHostToDevice));
+    return MS_OK;
+}
+
+extern "C" int ms_set_dial_clock(ms_ctx* c, int32_t s, double last_eq, double spacing, int64_t K, int64_t phase,
+                                 int32_t dialling) {
+    if (int r = shell_ok(c, s)) return r;
+    if (K <= 0) return fail(c, MS_ERR_ARG, "ms_set_dial_clock: steps_per_phase must be > 0");
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    S.dial_last_eq = last_eq;
+    S.dial_spacing = spacing;
+    S.dial_K = (double)K;
+    S.phase_steps = phase;
+    S.dialling = dialling ? 1 : 0;
+    return push_shells(c);
+}
+
+extern "C" int ms_get_dial_factor(ms_ctx* c, int32_t s, double* f, int64_t* phase) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = pull_shells(c)) return r;
+    const ShellDev& S = c->hs[s];
+    double d = S.dial_last_eq + S.dial_spacing * ((double)S.phase_steps / S.dial_K);
+    if (d >= 1.0) d = 1.0;
+    if (f) *f = d;
+    if (phase) *phase = S.phase_steps;
+    return MS_OK;
+}
+
+extern "C" int ms_set_shell_params(ms_ctx* c, int32_t s, const ms_shell_params* p) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!p) return fail(c, MS_ERR_ARG, "ms_set_shell_params: null params");
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    const int old_protocol = S.protocol;
+    params_to_dev(*p, S);
+    if (S.protocol != old_protocol) {
+        S.protocol = old_protocol;
+        return fail(c, MS_ERR_ARG, "ms_set_shell_params: the dial protocol is fixed at ms_create");
+    }
+    if (S.from_ansatz && !S.has_a0) {
+        // a0 = flat until ms_set_zero_dial_geometry is called
+    }
+    return push_shells(c);
+}
+
+extern "C" int ms_set_zero_dial_geometry(ms_ctx* c, int32_t s, const double* a0, const double* b0) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    const int T = c->T;
+    if (a0) {
+        if (!c->a0) {
+            std::vector<double> init(3 * (size_t)T);
+            for (int t = 0; t < T; ++t) {
+                init[t] = 1.0;
+                init[(size_t)T + t] = 0.0;
+                init[2 * (size_t)T + t] = 1.0;
+            }
+            HIPC(c, dalloc(c, &c->a0, init.size()));
+            HIPC(c, hipMemcpy(c->a0, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        for (int k = 0; k < 3; ++k)
+            HIPC(c, hipMemcpy(c->a0 + (size_t)k * T + S.tri_begin, a0 + (size_t)k * S.num_tris,
+                              sizeof(double) * S.num_tris, hipMemcpyHostToDevice));
+        S.has_a0 = 1;
+    }
+    if (b0) {
+        if (!c->b0) {
+            HIPC(c, dalloc(c, &c->b0, 3 * (size_t)T));
+            HIPC(c, hipMemset(c->b0, 0, 3 * sizeof(double) * (size_t)T));
+        }
+        for (int k = 0; k < 3; ++k)
+            HIPC(c, hipMemcpy(c->b0 + (size_t)k * T + S.tri_begin, b0 + (size_t)k * S.num_tris,
+                              sizeof(double) * S.num_tris, hipMemcpyHostToDevice));
+        S.has_b0 = 1;
+    }
+    destroy_graph(c);  // kernel arguments changed
+    return push_shells(c);
+}
+
+extern "C" int ms_set_precision(ms_ctx* c, int32_t precision) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    if (precision != MS_FP64 && precision != MS_MIXED) return fail(c, MS_ERR_ARG, "ms_set_precision: bad mode");
+    c->precision = precision;
+    return MS_OK;
+}
+
+extern "C" int ms_get_precision(const ms_ctx* c) { return c ? c->precision : -1; }
+
+// --------------------------------------------------------------- the ops
+
+extern "C" int ms_calc_deformation_forces(ms_ctx* c) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    const bool mixed = c->precision == MS_MIXED;
+    launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, false);
+    launch_node_gather(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, mixed);
+    HIPC(c, hipGetLastError());
+    return MS_OK;
+}
+
+extern "C" int ms_calc_non_deformation_forces(ms_ctx* c) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    launch_node_damping(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells);
+    HIPC(c, hipGetLastError());
+    return MS_OK;
+}
+
+extern "C" int ms_impose_constraints(ms_ctx* c) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    launch_node_clamp(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells);
+    HIPC(c, hipGetLastError());
+    return MS_OK;
+}
+
+namespace {
+int scan_bad(ms_ctx* c, std::vector<long long>& bad) {
+    launch_node_scan(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells,
+                     c->num_shells);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    bad.resize(c->num_shells);
+    HIPC(c, hipMemcpy(bad.data(), c->b.bad_dof, sizeof(long long) * c->num_shells, hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+int read_force(ms_ctx* c, int s, long long dof, double* out) {
+    const ShellDev& S = c->hs[s];
+    const long long comp = dof / S.num_nodes, n = dof % S.num_nodes;
+    HIPC(c, hipMemcpy(out, c->b.F + comp * c->N + S.node_begin + n, sizeof(double), hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+}  // namespace
+
+extern "C" int ms_advance_dynamics(ms_ctx* c, ms_step_report* reports) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    std::vector<long long> bad;
+    if (int r = scan_bad(c, bad)) return r;
+    launch_node_advance(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells,
+                        c->num_shells, c->d_clock);
+    HIPC(c, hipGetLastError());
+    c->steps += 1;
+    if (int r = pull_shells(c)) return r;
+    int status = MS_OK;
+    for (int s = 0; s < c->num_shells; ++s) {
+        ms_step_report rep{};
+        rep.fail_step = -1;
+        rep.bad_dof = -1;
+        if (bad[s] != LLONG_MAX) {
+            rep.status = MS_ERR_HIGH_FORCE;
+            rep.bad_dof = bad[s];
+            rep.fail_step = c->steps - 1;
+            if (int r = read_force(c, s, bad[s], &rep.bad_force)) return r;
+            status = MS_ERR_HIGH_FORCE;
+        }
+        rep.halted = c->hs[s].halted;
+        if (reports) reports[s] = rep;
+    }
+    if (status != MS_OK) c->err = "implausibly_high_force";
+    return status;
+}
+
+extern "C" int ms_is_at_equilibrium(ms_ctx* c, int32_t s, double tol_f, double tol_v, ms_step_report* rep) {
+    if (int r = shell_ok(c, s)) return r;
+    launch_node_equil(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells,
+                      c->num_shells, c->d_clock);
+    HIPC(c, hipGetLastError());
+    if (int r = pull_shells(c)) return r;
+    const ShellDev& S = c->hs[s];
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->fail_step = -1;
+        rep->bad_dof = -1;
+        rep->max_force_ratio = S.max_f;
+        rep->max_force_node = S.f_node;
+        rep->max_speed_ratio = S.max_v;
+        rep->max_speed_node = S.v_node;
+        rep->checked = 1;
+        rep->at_equilibrium = (S.max_f < tol_f && S.max_v < tol_v) ? 1 : 0;
+        rep->halted = S.halted;
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_calc_energies(ms_ctx* c, int32_t s, double out[3]) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!out) return fail(c, MS_ERR_ARG, "ms_calc_energies: null output");
+    launch_tri_energy(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, nullptr);
+    launch_reduce_partials(c->stream, c->b.part, c->d_shells, c->num_shells, true, 2, c->red_out);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double st[2];
+    HIPC(c, hipMemcpy(st, c->red_out + 2 * (size_t)s, sizeof(st), hipMemcpyDeviceToHost));
+    launch_node_kinetic(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells);
+    launch_reduce_partials(c->stream, c->b.part, c->d_shells, c->num_shells, false, 1, c->red_out);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double ke;
+    HIPC(c, hipMemcpy(&ke, c->red_out + s, sizeof(double), hipMemcpyDeviceToHost));
+    out[0] = st[0];
+    out[1] = st[1];
+    out[2] = ke;
+    return MS_OK;
+}
+
+namespace {
+int ensure_scratch(ms_ctx* c) {
+    if (!c->scratch3) HIPC(c, hipMalloc((void**)&c->scratch3, 9 * sizeof(double) * (size_t)c->T));
+    return MS_OK;
+}
+}  // namespace
+
+extern "C" int ms_calc_geometry(ms_ctx* c, int32_t s, double* a, double* b, double* n) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = ensure_scratch(c)) return r;
+    const size_t T = c->T;
+    double *ga = c->scratch3, *gb = c->scratch3 + 3 * T, *gn = c->scratch3 + 6 * T;
+    launch_tri_geometry(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, ga, gb, gn);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    double* outs[3] = {a, b, n};
+    double* srcs[3] = {ga, gb, gn};
+    for (int q = 0; q < 3; ++q) {
+        if (!outs[q]) continue;
+        for (int k = 0; k < 3; ++k)
+            HIPC(c, hipMemcpy(outs[q] + (size_t)k * S.num_tris, srcs[q] + k * T + S.tri_begin,
+                              sizeof(double) * S.num_tris, hipMemcpyDeviceToHost));
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_calc_energy_densities(ms_ctx* c, int32_t s, double* dens) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = ensure_scratch(c)) return r;
+    const size_t T = c->T;
+    launch_tri_energy(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, c->scratch3);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    if (dens)
+        for (int k = 0; k < 2; ++k)
+            HIPC(c, hipMemcpy(dens + (size_t)k * S.num_tris, c->scratch3 + k * T + S.tri_begin,
+                              sizeof(double) * S.num_tris, hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+
+// ------------------------------------------------------------ fused path
+
+extern "C" int ms_step(ms_ctx* c, int64_t nsteps, int64_t equil_every, double tol_f, double tol_v, int32_t halt_on_eq,
+                       ms_step_report* reports) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    if (nsteps < 0) return fail(c, MS_ERR_ARG, "ms_step: nsteps < 0");
+    if (int r = pull_shells(c)) return r;
+    std::vector<int> was_failed(c->num_shells);
+    for (int s = 0; s < c->num_shells; ++s) {
+        was_failed[s] = c->hs[s].failed;
+        c->hs[s].checked = 0;
+    }
+    if (int r = push_shells(c)) return r;
+    const bool use_graph = !c->timing;
+    if (use_graph)
+        if (int r = ensure_graph(c)) return r;
+    double t_tri = 0.0, t_node = 0.0;
+    long long n_timed = 0;
+    int64_t left = nsteps;
+    while (left > 0) {
+        int64_t plain = left;
+        bool check_now = false;
+        if (equil_every > 0) {
+            const int64_t to_check = equil_every - 1 - (c->steps % equil_every);
+            if (to_check == 0)
+                check_now = true;
+            else
+                plain = std::min<int64_t>(left, to_check);
+        }
+        if (check_now) {
+            step_once(c, true, halt_on_eq, tol_f, tol_v);
+            HIPC(c, hipGetLastError());
+            c->steps += 1;
+            left -= 1;
+            continue;
+        }
+        if (use_graph) {
+            while (plain >= c->graph_steps) {
+                HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
+                c->steps += c->graph_steps;
+                plain -= c->graph_steps;
+                left -= c->graph_steps;
+            }
+        }
+        while (plain > 0) {
+            if (c->timing) {
+                const bool mixed = c->precision == MS_MIXED;
+                hipEventRecord(c->ev[0], c->stream);
+                launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells,
+                                  mixed, true);
+                hipEventRecord(c->ev[1], c->stream);
+                StepFlags fl{0, 0, 0.0, 0.0};
+                launch_node_step(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks,
+                                 c->d_shells, c->num_shells, c->d_clock, mixed, fl);
+                hipEventRecord(c->ev[2], c->stream);
+                HIPC(c, hipEventSynchronize(c->ev[2]));
+                float a = 0, b = 0;
+                hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+                hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+                t_tri += a;
+                t_node += b;
+                ++n_timed;
+            } else {
+                step_once(c, false, 0, 0.0, 0.0);
+            }
+            HIPC(c, hipGetLastError());
+            c->steps += 1;
+            plain -= 1;
+            left -= 1;
+        }
+    }
+    if (n_timed > 0) {
+        c->last_ms[0] = t_tri / n_timed;
+        c->last_ms[1] = t_node / n_timed;
+        c->last_ms[2] = (t_tri + t_node) / n_timed;
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (int r = pull_shells(c)) return r;
+    // crash reports for shells that failed in this call: recompute the
+    // failing step's forces from the retained pre-step state
+    bool new_fail = false;
+    for (int s = 0; s < c->num_shells; ++s) new_fail |= (c->hs[s].failed && !was_failed[s]);
+    std::vector<long long> bad;
+    if (new_fail) {
+        if (int r = ms_calc_deformation_forces(c)) return r;
+        if (int r = ms_calc_non_deformation_forces(c)) return r;
+        if (int r = ms_impose_constraints(c)) return r;
+        if (int r = scan_bad(c, bad)) return r;
+    }
+    int status = MS_OK;
+    for (int s = 0; s < c->num_shells; ++s) {
+        const ShellDev& S = c->hs[s];
+        ms_step_report rep{};
+        rep.status = S.failed ? MS_ERR_HIGH_FORCE : MS_OK;
+        rep.halted = S.halted;
+        rep.fail_step = S.fail_step;
+        rep.bad_dof = -1;
+        if (S.failed && !was_failed[s]) {
+            status = MS_ERR_HIGH_FORCE;
+            if (bad[s] != LLONG_MAX) {
+                rep.bad_dof = bad[s];
+                if (int r = read_force(c, s, bad[s], &rep.bad_force)) return r;
+            }
+        }
+        rep.checked = S.checked;
+        rep.max_force_ratio = S.max_f;
+        rep.max_force_node = S.f_node;
+        rep.max_speed_ratio = S.max_v;
+        rep.max_speed_node = S.v_node;
+        rep.at_equilibrium = S.at_eq;
+        if (reports) reports[s] = rep;
+    }
+    if (status != MS_OK) c->err = "implausibly_high_force";
+    return status;
+}
+
+extern "C" int ms_resume_shell(ms_ctx* c, int32_t s) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    S.halted = 0;
+    S.failed = 0;
+    S.at_eq = 0;
+    S.halt_step = -1;
+    S.fail_step = -1;
+    return push_shells(c);
+}
+
+extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
+    if (!c || !t) return fail(c, MS_ERR_ARG, "ms_set_tuning: bad arguments");
+    ms_tuning n = c->tune;
+    auto okblock = [](int b) { return b == 64 || b == 128 || b == 256; };
+    if (t->tri_block) {
+        if (!okblock(t->tri_block)) return fail(c, MS_ERR_ARG, "tri_block must be 64, 128 or 256");
+        n.tri_block = t->tri_block;
+    }
+    if (t->node_block) {
+        if (!okblock(t->node_block)) return fail(c, MS_ERR_ARG, "node_block must be 64, 128 or 256");
+        n.node_block = t->node_block;
+    }
+    if (t->tri_grid_per_cu >= 0) n.tri_grid_per_cu = t->tri_grid_per_cu;
+    if (t->node_grid_per_cu >= 0) n.node_grid_per_cu = t->node_grid_per_cu;
+    if (t->graph_steps > 0) n.graph_steps = t->graph_steps;
+    const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
+    if (int r = pull_shells(c)) return r;
+    c->tune = n;
+    destroy_graph(c);
+    if (rechunk) {
+        if (int r = build_chunks(c)) return r;
+        if (int r = push_shells(c)) return r;
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_get_tuning(const ms_ctx* c, ms_tuning* t) {
+    if (!c || !t) return MS_ERR_ARG;
+    *t = c->tune;
+    return MS_OK;
+}
+
+extern "C" int ms_enable_timing(ms_ctx* c, int32_t on) {
+    if (!c) return MS_ERR_ARG;
+    c->timing = on != 0;
+    return MS_OK;
+}
+
+extern "C" int ms_synchronize(ms_ctx* c) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MS_OK;
+}
+
+extern "C" int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms) {
+    if (nbytes <= 0 || nbytes % 16 != 0 || reps <= 0)
+        return fail(nullptr, MS_ERR_ARG, "ms_calibration_copy: nbytes must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MS_OK;
+    if (hipMalloc(&a, (size_t)nbytes) != hipSuccess || hipMalloc(&b, (size_t)nbytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(nullptr, MS_ERR_HIP, "ms_calibration_copy: allocation failed");
+    } else {
+        hipMemsetAsync(a, 0, (size_t)nbytes, s);
+        launch_copy(s, a, b, nbytes);  // warm
+        hipEventRecord(e0, s);
+        for (int i = 0; i < reps; ++i) launch_copy(s, a, b, nbytes);
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(nullptr, MS_ERR_HIP, "ms_calibration_copy: kernel failed");
+        } else if (out_ms) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            *out_ms = ms / reps;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}
+
+extern "C" int ms_last_step_timing(const ms_ctx* c, double out[3]) {
+    if (!c || !out) return MS_ERR_ARG;
+    for (int i = 0; i < 3; ++i) out[i] = c->last_ms[i];
+    return MS_OK;
+}

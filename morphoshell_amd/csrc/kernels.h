@@ -1,0 +1,101 @@
+// Launch interface between the host context (context.cpp) and the device
+// kernels (tri_kernels.hip, node_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+
+namespace ms {
+
+// Per-triangle streams.  Index [k * T + t] for k-th component of triangle t
+// (SoA over all shells' triangles, T = total triangles of the context).
+struct TriData {
+    int T;
+    int N;                    // total nodes (stride of the position arrays)
+    const int* pid;           // 6T   patch node ids (global node index)
+    const double* G;          // 6T   [(k*3 + v) * T + t]
+    const double* H;          // 18T  [(n*3 + k) * T + t]
+    const float* G32;         // 6T   (mixed mode copy, may be null)
+    const float* H32;         // 18T
+    const double* ainfo;      // acols * T  dial info (blend: abar xx,xy,yy; LCE: c2, cs, s2, lam, -2nu)
+    const double* binfo;      // 3T   (blend)
+    const double* a0;         // 3T or null (dial from ansatz)
+    const double* b0;         // 3T or null
+    const double* h;          // T    def_thicknesses
+    const double* mu;         // T    def_shear_moduli
+    const double* area;       // T    ref_area
+    const float* hmuA32;      // 3T   (h, mu, area) in FP32 for mixed mode
+};
+
+struct NodeData {
+    int N;
+    const double* mass;       // N
+    const unsigned char* clamp;  // N (1 = constrained)
+    const long long* csr_off; // N + 1   patch-incidence CSR offsets
+    const int* csr_ent;       // entries t*6 + slot, ascending per node
+};
+
+struct Buffers {
+    double* x[2];             // 3N positions, two buffers (per-shell parity)
+    double* v[2];             // 3N velocities
+    double* F;                // 3N forces (ops path)
+    double* contrib;          // 18T FP64 per-slot force contributions
+    float* contrib32;         // 18T FP32 contributions (mixed mode)
+    double* part;             // per-chunk partial sums / maxima scratch
+    int* part_i;
+    long long* bad_dof;       // per shell, ops-path blow-up scan
+};
+
+enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2 };
+
+struct Launch {
+    int grid;
+    int block;
+    hipStream_t stream;
+};
+
+// Triangle launches.  skip_halted: ms_step semantics (halted shells skipped).
+void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells, bool mixed, bool skip_halted);
+// Energy densities (dens: 2T, may be null) and per-chunk double-double
+// partial sums of A*W_s and A*W_b into b.part[4*chunk .. +3].
+void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells, double* dens);
+void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, double* ga, double* gb, double* gn);
+
+// Node launches.
+struct StepFlags {
+    int check_equil;       // fused equilibrium reduction on this step
+    int halt_on_eq;
+    double tol_f, tol_v;
+};
+void launch_node_step(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                      ShellDev* shells, int num_shells, ClockDev* clock, bool mixed, const StepFlags& fl);
+void launch_node_gather(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                        ShellDev* shells, bool mixed);
+void launch_node_damping(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells);
+void launch_node_clamp(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells);
+// ms_advance_dynamics: blow-up scan into b.bad_dof (per shell, min dof) ...
+void launch_node_scan(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                      ShellDev* shells, int num_shells);
+// ... then the update of every shell whose bad_dof is none, with bookkeeping.
+void launch_node_advance(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, int num_shells, ClockDev* clock);
+// Equilibrium maxima from the F buffer into per-shell fields (no halting).
+void launch_node_equil(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells, int num_shells, ClockDev* clock);
+// Kinetic energy partials (double-double) into b.part[4*chunk + 0..1].
+void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells);
+// Per-shell compensated combine of chunk partials: out[shell*ncomp + c] for
+// components c < ncomp stored as (hi, lo) pairs at part[4*chunk + 2c].
+void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* shells, int num_shells, bool tri_chunks,
+                            int ncomp, double* out);
+// Plain 16-byte-vector copy of nbytes (traffic-counter calibration).
+void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
+
+}  // namespace ms

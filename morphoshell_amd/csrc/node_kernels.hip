@@ -1,0 +1,570 @@
+// Per-node kernels of the per-step path (gfx950).  Compiled with
+// -ffp-contract=off: the elementwise dynamics round exactly as the
+// reference's C++ statements do (one rounding per operation), so given the
+// same forces the update is bit-identical to the CPU path.
+//
+// Force assembly is a deterministic gather: node n sums the contributions
+// of every (triangle, patch slot) whose patch contains n, in ascending
+// (triangle, slot) order, from the patch-incidence CSR.  This is the row
+// structure of the reference's explicit transpose M^T
+// (calc_deformation_forces.cpp:185, main.cpp:168-174); vertex incidence
+// alone (calc_triangles_incident_on_nodes.cpp) would miss the non-vertex
+// patch nodes.
+//
+// The fused step launch then applies damping (calc_non_deformation_forces
+// .cpp:55-57), the clamp (impose_constraints.cpp:61-66), optionally the
+// equilibrium max norms (is_at_equilibrium.cpp:62-93), the blow-up test
+// !(|F| < 5e4 F_char) (advance_dynamics.cpp:58-73) and the symplectic-Euler
+// or gradient-descent update (advance_dynamics.cpp:76-93), writing the other
+// of two position/velocity buffers.  Its last block does the per-step
+// bookkeeping: buffer parity, step and dial clocks, equilibrium verdicts,
+// halting of failed shells.
+
+#include <limits.h>
+#include <math.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace ms {
+namespace {
+
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+    const int X = 8;
+    const int q = n / X, r = n % X;
+    const int j = i % X, local = i / X;
+    return j * q + (j < r ? j : r) + local;
+}
+
+template <typename T>
+__device__ __forceinline__ T aload(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void astore(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double h2, double l2) {
+    double s, e;
+    two_sum(hi, h2, s, e);
+    e = e + (lo + l2);
+    two_sum(s, e, hi, lo);
+}
+
+// (max value, first index) block reduction of NV pairs; result in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
+    __shared__ double sv[NV][16];
+    __shared__ int si[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double v2 = __shfl_down(v[k], off, 64);
+            const int i2 = __shfl_down(idx[k], off, 64);
+            if (lane < off && (v2 > v[k] || (v2 == v[k] && i2 < idx[k]))) {
+                v[k] = v2;
+                idx[k] = i2;
+            }
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            sv[k][wave] = v[k];
+            si[k][wave] = idx[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            v[k] = sv[k][0];
+            idx[k] = si[k][0];
+            for (int w = 1; w < nw; ++w)
+                if (sv[k][w] > v[k] || (sv[k][w] == v[k] && si[k][w] < idx[k])) {
+                    v[k] = sv[k][w];
+                    idx[k] = si[k][w];
+                }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void block_dd_reduce(double hi[NV], double lo[NV]) {
+    __shared__ double sh_hi[NV][16], sh_lo[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double h2 = __shfl_down(hi[k], off, 64);
+            const double l2 = __shfl_down(lo[k], off, 64);
+            if (lane < off) dd_add(hi[k], lo[k], h2, l2);
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            sh_hi[k][wave] = hi[k];
+            sh_lo[k][wave] = lo[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            hi[k] = sh_hi[k][0];
+            lo[k] = sh_lo[k][0];
+            for (int w = 1; w < nw; ++w) dd_add(hi[k], lo[k], sh_hi[k][w], sh_lo[k][w]);
+        }
+    }
+    __syncthreads();
+}
+
+struct NodeKArgs {
+    NodeData nd;
+    Buffers b;
+    const Chunk* chunks;
+    int nchunks;
+    ShellDev* shells;
+    int num_shells;
+    ClockDev* clock;
+    StepFlags fl;
+};
+
+// Sum of the patch contributions of node n (fixed CSR order).
+template <typename CT>
+__device__ __forceinline__ void gather(const NodeData& nd, const CT* contrib, int n, double& fx, double& fy,
+                                       double& fz) {
+    const long long e0 = nd.csr_off[n], e1 = nd.csr_off[n + 1];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (long long e = e0; e < e1; ++e) {
+        const CT* c = contrib + (size_t)nd.csr_ent[e] * 3;
+        ax = ax + (double)c[0];
+        ay = ay + (double)c[1];
+        az = az + (double)c[2];
+    }
+    fx = ax;
+    fy = ay;
+    fz = az;
+}
+
+// Damping term of one dof, evaluated as the reference's statement.
+__device__ __forceinline__ double damped(double f, double eta, double m, double v, double den) {
+    double t = -eta * m;
+    t = t * v;
+    t = t / den;
+    return f + t;
+}
+
+// The update of one dof (advance_dynamics.cpp:81-92).
+__device__ __forceinline__ void update(const ShellDev& s, double m, double f, double& v, double& x) {
+    if (!s.gd) {
+        v = v + (s.dt / m) * f;
+    } else {
+        v = v + (s.dt / m) * f;
+        v = (s.damp_den * f) / (s.eta * m);
+    }
+    x = x + s.dt * v;
+}
+
+__device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+
+// Last-block bookkeeping after a step-like launch.  MODE 0 = fused step,
+// 1 = ops advance (shells with a blow-up are not advanced), 2 = equilibrium
+// maxima only.
+template <int MODE, bool CHECK>
+__device__ void bookkeeping(const NodeKArgs& k) {
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(&k.clock->done, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const long long step = aload(&k.clock->steps);
+    for (int s = threadIdx.x; s < k.num_shells; s += blockDim.x) {
+        ShellDev* S = &k.shells[s];
+        const int halted = aload(&S->halted);
+        if (MODE == 0 && halted) continue;
+        if (CHECK) {
+            double fm = -1.0, vm = -1.0;
+            int fi = INT_MAX, vi = INT_MAX;
+            const int c0 = S->node_chunk_begin, c1 = c0 + S->node_chunk_count;
+            for (int c = c0; c < c1; ++c) {
+                const double f = aload(&k.b.part[2 * c]), v = aload(&k.b.part[2 * c + 1]);
+                const int fii = aload(&k.b.part_i[2 * c]), vii = aload(&k.b.part_i[2 * c + 1]);
+                if (f > fm) {
+                    fm = f;
+                    fi = fii;
+                }
+                if (v > vm) {
+                    vm = v;
+                    vi = vii;
+                }
+            }
+            const double fr = fm / S->fchar, vr = vm / S->cchar;
+            S->max_f = fr;
+            S->max_v = vr;
+            S->f_node = fi - S->node_begin;
+            S->v_node = vi - S->node_begin;
+            S->checked = 1;
+            S->at_eq = (fr < k.fl.tol_f && vr < k.fl.tol_v) ? 1 : 0;
+        }
+        if (MODE == 2) continue;
+        if (MODE == 0) {
+            if (aload(&S->failed)) {
+                S->halted = 1;
+                S->fail_step = step;
+                S->halt_step = step;
+                continue;
+            }
+            if (CHECK && k.fl.halt_on_eq && S->at_eq) {
+                S->halted = 1;
+                S->halt_step = step;
+                continue;
+            }
+        }
+        if (MODE == 1 && aload(&k.b.bad_dof[s]) != LLONG_MAX) continue;
+        S->par ^= 1;
+        if (S->dialling) S->phase_steps += 1;
+    }
+    if (threadIdx.x == 0) {
+        if (MODE != 2) astore(&k.clock->steps, step + 1);
+        astore(&k.clock->done, 0u);
+    }
+}
+
+// ----------------------------------------------------------- fused step
+template <bool MIXED, bool CHECK>
+__global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const int ci = xcd_remap(vi, k.nchunks);
+        const Chunk ch = k.chunks[ci];
+        const ShellDev& sh = k.shells[ch.z];
+        if (sh.halted) continue;  // block-uniform
+        const int n = ch.x + threadIdx.x;
+        double fnorm = -1.0, vnorm = -1.0;
+        int fidx = INT_MAX, vidx = INT_MAX;
+        if (n < ch.y) {
+            const int par = sh.par;
+            const double* X = k.b.x[par];
+            const double* V = k.b.v[par];
+            double* Xo = k.b.x[par ^ 1];
+            double* Vo = k.b.v[par ^ 1];
+            double fx, fy, fz;
+            if (MIXED)
+                gather(k.nd, k.b.contrib32, n, fx, fy, fz);
+            else
+                gather(k.nd, k.b.contrib, n, fx, fy, fz);
+            double vx = V[n], vy = V[N + n], vz = V[2 * N + n];
+            const double m = k.nd.mass[n];
+            if (!sh.gd) {
+                fx = damped(fx, sh.eta, m, vx, sh.damp_den);
+                fy = damped(fy, sh.eta, m, vy, sh.damp_den);
+                fz = damped(fz, sh.eta, m, vz, sh.damp_den);
+            }
+            if (k.nd.clamp[n]) fx = fy = fz = 0.0;
+            if (CHECK) {
+                fnorm = norm3(fx, fy, fz);
+                vnorm = norm3(vx, vy, vz);
+                fidx = vidx = n;
+            }
+            const double lim = sh.lim;
+            if (!(fabs(fx) < lim) || !(fabs(fy) < lim) || !(fabs(fz) < lim))
+                atomicOr(&k.shells[ch.z].failed, 1);
+            double x = X[n], y = X[N + n], z = X[2 * N + n];
+            update(sh, m, fx, vx, x);
+            update(sh, m, fy, vy, y);
+            update(sh, m, fz, vz, z);
+            Xo[n] = x;
+            Xo[N + n] = y;
+            Xo[2 * N + n] = z;
+            Vo[n] = vx;
+            Vo[N + n] = vy;
+            Vo[2 * N + n] = vz;
+        }
+        if (CHECK) {
+            double v2[2] = {fnorm, vnorm};
+            int i2[2] = {fidx, vidx};
+            block_argmax<2>(v2, i2);
+            if (threadIdx.x == 0) {
+                k.b.part[2 * ci] = v2[0];
+                k.b.part[2 * ci + 1] = v2[1];
+                k.b.part_i[2 * ci] = i2[0];
+                k.b.part_i[2 * ci + 1] = i2[1];
+            }
+        }
+    }
+    bookkeeping<0, CHECK>(k);
+}
+
+// ------------------------------------------------------------ ops kernels
+template <bool MIXED>
+__global__ __launch_bounds__(256) void node_gather_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y) continue;
+        double fx, fy, fz;
+        if (MIXED)
+            gather(k.nd, k.b.contrib32, n, fx, fy, fz);
+        else
+            gather(k.nd, k.b.contrib, n, fx, fy, fz);
+        k.b.F[n] = fx;
+        k.b.F[N + n] = fy;
+        k.b.F[2 * N + n] = fz;
+    }
+}
+
+__global__ __launch_bounds__(256) void node_damping_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const ShellDev& sh = k.shells[ch.z];
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y || sh.gd) continue;
+        const double* V = k.b.v[sh.par];
+        const double m = k.nd.mass[n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) k.b.F[c * N + n] = damped(k.b.F[c * N + n], sh.eta, m, V[c * N + n], sh.damp_den);
+    }
+}
+
+__global__ __launch_bounds__(256) void node_clamp_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y || !k.nd.clamp[n]) continue;
+        k.b.F[n] = 0.0;
+        k.b.F[N + n] = 0.0;
+        k.b.F[2 * N + n] = 0.0;
+    }
+}
+
+__global__ void fill_ll_kernel(long long* p, int n, long long v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// First offending dof (n_local + c * N_shell) per shell, deterministic min.
+__global__ __launch_bounds__(256) void node_scan_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const ShellDev& sh = k.shells[ch.z];
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y) continue;
+        const double lim = sh.lim;
+        for (int c = 0; c < 3; ++c) {
+            const double f = k.b.F[c * N + n];
+            if (!(fabs(f) < lim)) {
+                const long long d = (long long)(n - sh.node_begin) + (long long)c * sh.num_nodes;
+                atomicMin(&k.b.bad_dof[ch.z], d);
+                break;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void node_advance_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const ShellDev& sh = k.shells[ch.z];
+        if (k.b.bad_dof[ch.z] != LLONG_MAX) continue;
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y) continue;
+        const int par = sh.par;
+        const double m = k.nd.mass[n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = k.b.v[par][c * N + n], x = k.b.x[par][c * N + n];
+            update(sh, m, k.b.F[c * N + n], v, x);
+            k.b.v[par ^ 1][c * N + n] = v;
+            k.b.x[par ^ 1][c * N + n] = x;
+        }
+    }
+    bookkeeping<1, false>(k);
+}
+
+__global__ __launch_bounds__(256) void node_equil_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const int ci = xcd_remap(vi, k.nchunks);
+        const Chunk ch = k.chunks[ci];
+        const ShellDev& sh = k.shells[ch.z];
+        const int n = ch.x + threadIdx.x;
+        double v2[2] = {-1.0, -1.0};
+        int i2[2] = {INT_MAX, INT_MAX};
+        if (n < ch.y) {
+            const double* V = k.b.v[sh.par];
+            v2[0] = norm3(k.b.F[n], k.b.F[N + n], k.b.F[2 * N + n]);
+            v2[1] = norm3(V[n], V[N + n], V[2 * N + n]);
+            i2[0] = i2[1] = n;
+        }
+        block_argmax<2>(v2, i2);
+        if (threadIdx.x == 0) {
+            k.b.part[2 * ci] = v2[0];
+            k.b.part[2 * ci + 1] = v2[1];
+            k.b.part_i[2 * ci] = i2[0];
+            k.b.part_i[2 * ci + 1] = i2[1];
+        }
+    }
+    bookkeeping<2, true>(k);
+}
+
+__global__ __launch_bounds__(256) void node_kinetic_kernel(NodeKArgs k) {
+    const int N = k.nd.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const int ci = xcd_remap(vi, k.nchunks);
+        const Chunk ch = k.chunks[ci];
+        const ShellDev& sh = k.shells[ch.z];
+        const int n = ch.x + threadIdx.x;
+        double hi[1] = {0.0}, lo[1] = {0.0};
+        if (n < ch.y) {
+            const double* V = k.b.v[sh.par];
+            const double vx = V[n], vy = V[N + n], vz = V[2 * N + n];
+            hi[0] = (0.5 * k.nd.mass[n]) * (vx * vx + vy * vy + vz * vz);
+        }
+        block_dd_reduce<1>(hi, lo);
+        if (threadIdx.x == 0) {
+            k.b.part[4 * ci] = hi[0];
+            k.b.part[4 * ci + 1] = lo[0];
+        }
+    }
+}
+
+__global__ void reduce_partials_kernel(const double* part, const ShellDev* shells, int num_shells, int tri_chunks,
+                                       int ncomp, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= num_shells) return;
+    const ShellDev& S = shells[s];
+    const int c0 = tri_chunks ? S.tri_chunk_begin : S.node_chunk_begin;
+    const int cn = tri_chunks ? S.tri_chunk_count : S.node_chunk_count;
+    for (int comp = 0; comp < ncomp; ++comp) {
+        double hi = 0.0, lo = 0.0;
+        for (int c = c0; c < c0 + cn; ++c) dd_add(hi, lo, part[4 * c + 2 * comp], part[4 * c + 2 * comp + 1]);
+        out[s * ncomp + comp] = hi + lo;
+    }
+}
+
+NodeKArgs make_args(const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks, ShellDev* shells,
+                    int num_shells, ClockDev* clock) {
+    NodeKArgs k{};
+    k.nd = nd;
+    k.b = b;
+    k.chunks = chunks;
+    k.nchunks = nchunks;
+    k.shells = shells;
+    k.num_shells = num_shells;
+    k.clock = clock;
+    return k;
+}
+
+}  // namespace
+
+void launch_node_step(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                      ShellDev* shells, int num_shells, ClockDev* clock, bool mixed, const StepFlags& fl) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, clock);
+    k.fl = fl;
+    const dim3 g(L.grid), bl(L.block);
+    if (mixed) {
+        if (fl.check_equil)
+            hipLaunchKernelGGL((node_step_kernel<true, true>), g, bl, 0, L.stream, k);
+        else
+            hipLaunchKernelGGL((node_step_kernel<true, false>), g, bl, 0, L.stream, k);
+    } else {
+        if (fl.check_equil)
+            hipLaunchKernelGGL((node_step_kernel<false, true>), g, bl, 0, L.stream, k);
+        else
+            hipLaunchKernelGGL((node_step_kernel<false, false>), g, bl, 0, L.stream, k);
+    }
+}
+
+void launch_node_gather(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                        ShellDev* shells, bool mixed) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
+    if (mixed)
+        hipLaunchKernelGGL(node_gather_kernel<true>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    else
+        hipLaunchKernelGGL(node_gather_kernel<false>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_damping(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
+    hipLaunchKernelGGL(node_damping_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_clamp(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
+    hipLaunchKernelGGL(node_clamp_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_scan(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                      ShellDev* shells, int num_shells) {
+    hipLaunchKernelGGL(fill_ll_kernel, dim3((num_shells + 255) / 256), dim3(256), 0, L.stream, b.bad_dof, num_shells,
+                       (long long)LLONG_MAX);
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, nullptr);
+    hipLaunchKernelGGL(node_scan_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_advance(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, int num_shells, ClockDev* clock) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, clock);
+    hipLaunchKernelGGL(node_advance_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_equil(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                       ShellDev* shells, int num_shells, ClockDev* clock) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, clock);
+    k.fl.tol_f = 0.0;
+    k.fl.tol_v = 0.0;
+    hipLaunchKernelGGL(node_equil_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
+    hipLaunchKernelGGL(node_kinetic_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* shells, int num_shells, bool tri_chunks,
+                            int ncomp, double* out) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((num_shells + 63) / 64), dim3(64), 0, s, part, shells, num_shells,
+                       tri_chunks ? 1 : 0, ncomp, out);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                   long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes) {
+    const long long n = nbytes / 16;
+    const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
+}
+
+}  // namespace ms

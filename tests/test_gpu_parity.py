@@ -1,0 +1,302 @@
+"""GPU parity of the per-step path against the FP64 oracle, through the C-ABI.
+
+Tolerances (stated per test):
+  * FP64 forces: max |F_gpu - F_orc| <= 1e-10 * max |F_orc|;
+  * FP64 energies: 1e-11 relative; geometry 1e-12 (a) / 1e-9 (b, from H ~ 1/s^2);
+  * elementwise dynamics given the same forces: bit-identical;
+  * mixed mode: force error <= 2e-4 * max |F| (FP32 patch fit and stresses),
+    printed for the record.
+"""
+import numpy as np
+import pytest
+
+from conftest import deformed_state, grf_disc
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _shell(spacing=0.04, seed=0, h=0.01, kappa=0.5, gamma=40.0, clamp_every=0, protocol=1, **kw):
+    from morphoshell_amd.meshing import lce_defect_abar_info
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    m = grf_disc(spacing, seed=seed, h=h, kappa=kappa)
+    if protocol == 3:
+        m.abar_info = lce_defect_abar_info(m.centroids(), 0.9, 0.5)
+    if clamp_every:
+        cl = np.zeros(m.num_nodes, np.uint8)
+        cl[::clamp_every] = 1
+        m.constrained = cl
+    st = Settings(bend_stiffening_scale_factor=gamma, dial_protocol=protocol, **kw)
+    return prepare_shell(m, st)
+
+
+@pytest.fixture(scope="module")
+def sh():
+    return _shell(clamp_every=11)
+
+
+def _ctx(shells, precision="fp64"):
+    from morphoshell_amd.shell import Context
+
+    return Context(shells, precision=precision)
+
+
+def test_forces_fp64_match_oracle(sh):
+    x, v = deformed_state(sh, seed=3)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.calc_deformation_forces()
+        Fg = ctx.get_forces(0)
+    Fo = orc.System.from_shell(sh).deformation_forces(x)
+    err = np.abs(Fg - Fo).max() / np.abs(Fo).max()
+    print(f"FP64 force rel err {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_full_force_damping_clamp_match_oracle(sh):
+    x, v = deformed_state(sh, seed=4)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.calc_deformation_forces()
+        ctx.calc_non_deformation_forces()
+        ctx.impose_constraints()
+        Fg = ctx.get_forces(0)
+    Fo = orc.System.from_shell(sh).full_forces(x, v)
+    assert np.abs(Fg - Fo).max() <= 1e-10 * np.abs(Fo).max()
+    cl = sh.mesh.constrained.astype(bool)
+    assert np.all(Fg.reshape(3, -1)[:, cl] == 0)
+
+
+def test_damping_clamp_and_advance_bitwise(sh):
+    """Given identical forces, damping, clamp and update round exactly as the CPU path."""
+    x, v = deformed_state(sh, seed=5)
+    rng = np.random.default_rng(0)
+    f = rng.normal(size=3 * sh.N) * sh.scales["char_force_scale"]
+    sysm = orc.System.from_shell(sh)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.set_forces(0, f)
+        ctx.calc_non_deformation_forces()
+        ctx.impose_constraints()
+        fg = ctx.get_forces(0)
+        fo = f.copy()
+        sysm.damping(v, fo)
+        sysm.constraints(fo)
+        assert np.array_equal(fg, fo)
+        ctx.advance_dynamics()
+        xg, vg = ctx.get_state(0)
+    xo, vo = x.copy(), v.copy()
+    d, _ = sysm.advance(xo, vo, fo)
+    assert d == -1
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo)
+
+
+def test_energies_and_densities_match_oracle(sh):
+    x, v = deformed_state(sh, seed=6)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        Eg = ctx.calc_energies(0)
+        Dg = ctx.calc_energy_densities(0)
+    sysm = orc.System.from_shell(sh)
+    Eo = sysm.energies(x, v)
+    Do = sysm.energy_densities(x)
+    print("energies gpu", Eg, "oracle", Eo)
+    assert np.allclose(Eg, Eo, rtol=1e-11, atol=0)
+    assert np.abs(Dg - Do).max() <= 1e-10 * np.abs(Do).max()
+
+
+def test_geometry_matches_oracle(sh):
+    x, _ = deformed_state(sh, seed=7)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x)
+        a, b, n = ctx.calc_a_comps_and_b_comps_and_normals(0)
+    ao, bo, no = orc.System.from_shell(sh).geometry(x)
+    assert np.abs(a - ao).max() <= 1e-12
+    assert np.abs(n - no).max() <= 1e-12
+    assert np.abs(b - bo).max() <= 1e-9 * max(1.0, np.abs(bo).max())
+
+
+def test_equilibrium_report_matches_oracle(sh):
+    x, v = deformed_state(sh, seed=8)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.calc_deformation_forces()
+        ctx.calc_non_deformation_forces()
+        ctx.impose_constraints()
+        Fg = ctx.get_forces(0)
+        rep = ctx.is_at_equilibrium(0)
+    fr, fi, vr, vi = orc.System.from_shell(sh).equilibrium(Fg, v)
+    assert rep["max_force_node"] == fi and rep["max_speed_node"] == vi
+    assert rep["max_force_ratio"] == fr and rep["max_speed_ratio"] == vr
+
+
+def test_fused_step_equals_ops_sequence_bitwise(sh):
+    x, v = deformed_state(sh, seed=9)
+    with _ctx(sh) as a, _ctx(sh) as b:
+        a.set_state(0, x, v)
+        b.set_state(0, x, v)
+        for _ in range(3):
+            a.calc_deformation_forces()
+            a.calc_non_deformation_forces()
+            a.impose_constraints()
+            a.advance_dynamics()
+        b.step(3)
+        xa, va = a.get_state(0)
+        xb, vb = b.get_state(0)
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def test_trajectory_matches_oracle_and_is_deterministic(sh):
+    x, v = deformed_state(sh, seed=10)
+    n = 200
+    runs = []
+    for _ in range(2):
+        with _ctx(sh) as ctx:
+            ctx.set_state(0, x, v)
+            ctx.step(n)
+            runs.append(ctx.get_state(0))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    sysm = orc.System.from_shell(sh)
+    xo, vo = x.copy(), v.copy()
+    assert sysm.run_steps(xo, vo, n) == -1
+    dx = np.abs(runs[0][0] - xo).max()
+    print(f"200-step position deviation {dx:.3e}")
+    assert dx <= 1e-11
+
+
+def test_graph_replay_equals_direct_launches(sh):
+    x, v = deformed_state(sh, seed=11)
+    with _ctx(sh) as a, _ctx(sh) as b:
+        a.set_state(0, x, v)
+        b.set_state(0, x, v)
+        a.set_tuning(graph_steps=8)
+        a.step(37)          # 4 graph replays + 5 direct steps
+        b.enable_timing(True)
+        b.step(37)          # all direct, with events
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+        t = b.last_step_timing()
+        assert t[0] > 0 and t[1] > 0
+
+
+@pytest.mark.parametrize("tri_block,node_block,grid", [(64, 128, 0), (128, 64, 2), (256, 256, 1)])
+def test_launch_shapes_do_not_change_results(sh, tri_block, node_block, grid):
+    x, v = deformed_state(sh, seed=12)
+    with _ctx(sh) as a, _ctx(sh) as b:
+        a.set_state(0, x, v)
+        b.set_state(0, x, v)
+        b.set_tuning(tri_block=tri_block, node_block=node_block, tri_grid_per_cu=grid, node_grid_per_cu=grid)
+        a.step(5)
+        b.step(5)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+
+
+def test_blowup_reports_first_offending_dof(sh):
+    from morphoshell_amd.shell import HighForceError
+
+    x, v = deformed_state(sh, seed=13)
+    N = sh.N
+    free = np.where(sh.mesh.constrained == 0)[0]
+    k1, k2 = free[40], free[300]
+    x[k1] += 0.5   # displaced far beyond an edge length
+    x[2 * N + k2] += 0.5
+    sysm = orc.System.from_shell(sh)
+    fo = sysm.full_forces(x, v)
+    d_o, bad_o = sysm.advance(x.copy(), v.copy(), fo)
+    assert d_o >= 0
+    for fused in (False, True):
+        with _ctx(sh) as ctx:
+            ctx.set_state(0, x, v)
+            with pytest.raises(HighForceError) as ei:
+                if fused:
+                    ctx.step(4)
+                else:
+                    ctx.calc_deformation_forces()
+                    ctx.calc_non_deformation_forces()
+                    ctx.impose_constraints()
+                    ctx.advance_dynamics()
+            assert ei.value.dof == d_o
+            assert str(ei.value) == "implausibly_high_force"
+            assert abs(ei.value.force - bad_o) <= 1e-10 * abs(bad_o)
+            # nothing advanced: the retained state is the failing step's input
+            xs, _ = ctx.get_state(0)
+            assert np.array_equal(xs, x)
+
+
+def test_batch_of_shells_equals_single_shells():
+    shells = [_shell(0.08, seed=1), _shell(0.05, seed=2, kappa=-1.0), _shell(0.11, seed=3, h=0.02)]
+    states = [deformed_state(s, seed=20 + i) for i, s in enumerate(shells)]
+    with _ctx(shells) as ctx:
+        for i, (x, v) in enumerate(states):
+            ctx.set_state(i, x, v)
+        ctx.step(25)
+        batch = [ctx.get_state(i) for i in range(3)]
+        E = [ctx.calc_energies(i) for i in range(3)]
+    for i, s in enumerate(shells):
+        with _ctx(s) as c1:
+            c1.set_state(0, *states[i])
+            c1.step(25)
+            xs, vs = c1.get_state(0)
+            assert np.array_equal(xs, batch[i][0]) and np.array_equal(vs, batch[i][1])
+            assert np.allclose(c1.calc_energies(0), E[i], rtol=1e-14, atol=0)
+
+
+def test_device_dialling_matches_oracle():
+    for protocol in (1, 3):
+        s = _shell(0.06, seed=4, protocol=protocol)
+        x, v = deformed_state(s, seed=30)
+        f = 0.37
+        with _ctx(s) as ctx:
+            ctx.set_state(0, x, v)
+            K = 1000
+            ctx.set_dial_clock(0, 0.0, 1.0, K, int(f * K), True)
+            assert ctx.dial_factor(0) == pytest.approx(f)
+            ctx.calc_deformation_forces()
+            Fg = ctx.get_forces(0)
+        ab, bb = orc.dial(s.T, protocol, False, f, s.mesh.abar_info, s.mesh.bbar_info if protocol == 1 else None)
+        Fo = orc.System.from_shell(s, abar=ab, bbar=bb).deformation_forces(x)
+        assert np.abs(Fg - Fo).max() <= 1e-10 * np.abs(Fo).max(), protocol
+
+
+def test_dial_clock_advances_per_step():
+    s = _shell(0.1, seed=5)
+    x, v = deformed_state(s, seed=31)
+    with _ctx(s) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.set_dial_clock(0, 0.2, 0.5, 100, 10, True)
+        ctx.step(30)
+        assert ctx.dial_factor(0) == pytest.approx(0.2 + 0.5 * 40 / 100)
+        ctx.set_dial_clock(0, 0.2, 0.5, 100, 40, False)
+        ctx.step(5)
+        assert ctx.dial_factor(0) == pytest.approx(0.2 + 0.5 * 40 / 100)
+
+
+def test_mixed_precision_force_error(sh):
+    x, v = deformed_state(sh, seed=14)
+    with _ctx(sh, "mixed") as ctx:
+        ctx.set_state(0, x, v)
+        ctx.calc_deformation_forces()
+        Fm = ctx.get_forces(0)
+        ctx.set_precision("fp64")
+        ctx.calc_deformation_forces()
+        Fd = ctx.get_forces(0)
+    Fo = orc.System.from_shell(sh).deformation_forces(x)
+    em = np.abs(Fm - Fo).max() / np.abs(Fo).max()
+    ed = np.abs(Fd - Fo).max() / np.abs(Fo).max()
+    print(f"mixed force rel err {em:.3e}, fp64 {ed:.3e}")
+    assert ed <= 1e-10
+    assert em <= 2e-4
+
+
+def test_fused_equilibrium_check_and_halt():
+    s = _shell(0.1, seed=6, kappa=0.0)
+    # the flat reference state with abar = I, bbar = 0 is an equilibrium
+    s.mesh.abar_info[:] = [1.0, 0.0, 1.0]
+    with _ctx(s) as ctx:
+        ctx.set_state(0, s.ref_pos)
+        reps = ctx.step(10, equil_every=5, force_tol=1e-6, speed_tol=1e-5, halt_on_equilibrium=True)
+        r = reps[0]
+        assert r["checked"] and r["at_equilibrium"] and r["halted"]
+        assert r["max_force_ratio"] < 1e-10
+        x, _ = ctx.get_state(0)
+        assert np.array_equal(x, s.ref_pos)
