@@ -16,9 +16,9 @@
 // equilibrium max norms (is_at_equilibrium.cpp:62-93), the blow-up test
 // !(|F| < 5e4 F_char) (advance_dynamics.cpp:58-73) and the symplectic-Euler
 // or gradient-descent update (advance_dynamics.cpp:76-93), writing the other
-// of two position/velocity buffers.  Its last block does the per-step
-// bookkeeping: buffer parity, step and dial clocks, equilibrium verdicts,
-// halting of failed shells.
+// of two position/velocity buffers.  A one-block launch after it does the
+// per-step bookkeeping: buffer parity, step and dial clocks, equilibrium
+// verdicts, halting of failed shells.
 
 #include <limits.h>
 #include <math.h>
@@ -35,15 +35,6 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
     const int q = n / X, r = n % X;
     const int j = i % X, local = i / X;
     return j * q + (j < r ? j : r) + local;
-}
-
-template <typename T>
-__device__ __forceinline__ T aload(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void astore(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
@@ -148,7 +139,29 @@ __device__ __forceinline__ void gather(const NodeData& nd, const CT* contrib, in
                                        double& fz) {
     const long long e0 = nd.csr_off[n], e1 = nd.csr_off[n + 1];
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (long long e = e0; e < e1; ++e) {
+    long long e = e0;
+    // four entries' loads in flight at once; the sums keep CSR order
+    for (; e + 4 <= e1; e += 4) {
+        const CT* c0 = contrib + (size_t)nd.csr_ent[e] * 3;
+        const CT* c1 = contrib + (size_t)nd.csr_ent[e + 1] * 3;
+        const CT* c2 = contrib + (size_t)nd.csr_ent[e + 2] * 3;
+        const CT* c3 = contrib + (size_t)nd.csr_ent[e + 3] * 3;
+        const double x0 = c0[0], y0 = c0[1], z0 = c0[2], x1 = c1[0], y1 = c1[1], z1 = c1[2];
+        const double x2 = c2[0], y2 = c2[1], z2 = c2[2], x3 = c3[0], y3 = c3[1], z3 = c3[2];
+        ax = ax + x0;
+        ay = ay + y0;
+        az = az + z0;
+        ax = ax + x1;
+        ay = ay + y1;
+        az = az + z1;
+        ax = ax + x2;
+        ay = ay + y2;
+        az = az + z2;
+        ax = ax + x3;
+        ay = ay + y3;
+        az = az + z3;
+    }
+    for (; e < e1; ++e) {
         const CT* c = contrib + (size_t)nd.csr_ent[e] * 3;
         ax = ax + (double)c[0];
         ay = ay + (double)c[1];
@@ -180,33 +193,25 @@ __device__ __forceinline__ void update(const ShellDev& s, double m, double f, do
 
 __device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
 
-// Last-block bookkeeping after a step-like launch.  MODE 0 = fused step,
-// 1 = ops advance (shells with a blow-up are not advanced), 2 = equilibrium
-// maxima only.
+// Per-step bookkeeping, one small launch after a step-like node launch (the
+// kernel boundary makes the node launch's partials and flags visible, so no
+// grid-wide fences are needed).  MODE 0 = fused step, 1 = ops advance (shells
+// with a blow-up are not advanced), 2 = equilibrium maxima only.
 template <int MODE, bool CHECK>
-__device__ void bookkeeping(const NodeKArgs& k) {
-    __shared__ int last;
-    __threadfence();
+__global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
+    const long long step = k.clock->steps;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(&k.clock->done, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const long long step = aload(&k.clock->steps);
     for (int s = threadIdx.x; s < k.num_shells; s += blockDim.x) {
         ShellDev* S = &k.shells[s];
-        const int halted = aload(&S->halted);
+        const int halted = S->halted;
         if (MODE == 0 && halted) continue;
         if (CHECK) {
             double fm = -1.0, vm = -1.0;
             int fi = INT_MAX, vi = INT_MAX;
             const int c0 = S->node_chunk_begin, c1 = c0 + S->node_chunk_count;
             for (int c = c0; c < c1; ++c) {
-                const double f = aload(&k.b.part[2 * c]), v = aload(&k.b.part[2 * c + 1]);
-                const int fii = aload(&k.b.part_i[2 * c]), vii = aload(&k.b.part_i[2 * c + 1]);
+                const double f = k.b.part[2 * c], v = k.b.part[2 * c + 1];
+                const int fii = k.b.part_i[2 * c], vii = k.b.part_i[2 * c + 1];
                 if (f > fm) {
                     fm = f;
                     fi = fii;
@@ -226,7 +231,7 @@ __device__ void bookkeeping(const NodeKArgs& k) {
         }
         if (MODE == 2) continue;
         if (MODE == 0) {
-            if (aload(&S->failed)) {
+            if (S->failed) {
                 S->halted = 1;
                 S->fail_step = step;
                 S->halt_step = step;
@@ -238,14 +243,12 @@ __device__ void bookkeeping(const NodeKArgs& k) {
                 continue;
             }
         }
-        if (MODE == 1 && aload(&k.b.bad_dof[s]) != LLONG_MAX) continue;
+        if (MODE == 1 && k.b.bad_dof[s] != LLONG_MAX) continue;
         S->par ^= 1;
         if (S->dialling) S->phase_steps += 1;
     }
-    if (threadIdx.x == 0) {
-        if (MODE != 2) astore(&k.clock->steps, step + 1);
-        astore(&k.clock->done, 0u);
-    }
+    __syncthreads();
+    if (threadIdx.x == 0 && MODE != 2) k.clock->steps = step + 1;
 }
 
 // ----------------------------------------------------------- fused step
@@ -310,7 +313,6 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
             }
         }
     }
-    bookkeeping<0, CHECK>(k);
 }
 
 // ------------------------------------------------------------ ops kernels
@@ -401,7 +403,6 @@ __global__ __launch_bounds__(256) void node_advance_kernel(NodeKArgs k) {
             k.b.x[par ^ 1][c * N + n] = x;
         }
     }
-    bookkeeping<1, false>(k);
 }
 
 __global__ __launch_bounds__(256) void node_equil_kernel(NodeKArgs k) {
@@ -427,7 +428,6 @@ __global__ __launch_bounds__(256) void node_equil_kernel(NodeKArgs k) {
             k.b.part_i[2 * ci + 1] = i2[1];
         }
     }
-    bookkeeping<2, true>(k);
 }
 
 __global__ __launch_bounds__(256) void node_kinetic_kernel(NodeKArgs k) {
@@ -496,6 +496,10 @@ void launch_node_step(const Launch& L, const NodeData& nd, const Buffers& b, con
         else
             hipLaunchKernelGGL((node_step_kernel<false, false>), g, bl, 0, L.stream, k);
     }
+    if (fl.check_equil)
+        hipLaunchKernelGGL((bookkeeping_kernel<0, true>), dim3(1), dim3(256), 0, L.stream, k);
+    else
+        hipLaunchKernelGGL((bookkeeping_kernel<0, false>), dim3(1), dim3(256), 0, L.stream, k);
 }
 
 void launch_node_gather(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
@@ -531,6 +535,7 @@ void launch_node_advance(const Launch& L, const NodeData& nd, const Buffers& b, 
                          ShellDev* shells, int num_shells, ClockDev* clock) {
     NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, clock);
     hipLaunchKernelGGL(node_advance_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    hipLaunchKernelGGL((bookkeeping_kernel<1, false>), dim3(1), dim3(256), 0, L.stream, k);
 }
 
 void launch_node_equil(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
@@ -539,6 +544,7 @@ void launch_node_equil(const Launch& L, const NodeData& nd, const Buffers& b, co
     k.fl.tol_f = 0.0;
     k.fl.tol_v = 0.0;
     hipLaunchKernelGGL(node_equil_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    hipLaunchKernelGGL((bookkeeping_kernel<2, true>), dim3(1), dim3(256), 0, L.stream, k);
 }
 
 void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,

@@ -292,11 +292,23 @@ def test_fused_equilibrium_check_and_halt():
     s = _shell(0.1, seed=6, kappa=0.0)
     # the flat reference state with abar = I, bbar = 0 is an equilibrium
     s.mesh.abar_info[:] = [1.0, 0.0, 1.0]
-    with _ctx(s) as ctx:
+    with _ctx(s) as ctx, _ctx(s) as plain:
         ctx.set_state(0, s.ref_pos)
+        plain.set_state(0, s.ref_pos)
+        # checks run on clock steps 4, 9, ...: the shell halts at step 4 and
+        # keeps the state that was checked (the input of step 4)
         reps = ctx.step(10, equil_every=5, force_tol=1e-6, speed_tol=1e-5, halt_on_equilibrium=True)
         r = reps[0]
         assert r["checked"] and r["at_equilibrium"] and r["halted"]
         assert r["max_force_ratio"] < 1e-10
-        x, _ = ctx.get_state(0)
-        assert np.array_equal(x, s.ref_pos)
+        plain.step(4)
+        x, v = ctx.get_state(0)
+        xp, vp = plain.get_state(0)
+        assert np.array_equal(x, xp) and np.array_equal(v, vp)
+        # a halted shell is skipped by further steps
+        ctx.step(7)
+        assert np.array_equal(ctx.get_state(0)[0], xp)
+        ctx.resume(0)
+        ctx.step(1)
+        plain.step(1)
+        assert np.array_equal(ctx.get_state(0)[0], plain.get_state(0)[0])
