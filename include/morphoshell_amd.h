@@ -265,6 +265,46 @@ int ms_step(ms_ctx* ctx, int64_t nsteps, int64_t equil_every, double force_tol, 
             int32_t halt_on_equilibrium, ms_step_report* reports);
 /* Steps taken so far by the context clock. */
 int64_t ms_steps_taken(const ms_ctx* ctx);
+
+/* ------------------------------------------------- device-run relaxation
+ * The reference's dynamics-loop state machine (main.cpp:269-507) run on the
+ * device for each shell: dialling (damping eta1) for steps_per_dial_phase
+ * steps, then waiting_for_equil (damping eta2) with an equilibrium check
+ * every equil_interval steps (is_at_equilibrium.cpp:92-93: both ratios below
+ * their thresholds), then the next dialling phase from the dial factor just
+ * equilibrated at.  A shell halts at the first equilibrium with dial factor 1
+ * (the reference's loop would re-dial forever, main.cpp:493), or at every
+ * equilibrium if pause_on_eq is set, keeping the checked state. */
+typedef struct ms_relax_params {
+    double tol_f;                 /* force_to_char_force_ratio_equil_threshold */
+    double tol_v;                 /* speed_to_char_speed_ratio_equil_threshold */
+    int64_t equil_interval;       /* timesteps_between_equil_checks */
+    int64_t steps_per_dial_phase; /* timesteps_per_dial_phase (> 0) */
+    double dial_spacing;          /* dial_factors_to_hold_at_spacing */
+    double dial_start;            /* dial factor of the first step */
+    double eta1, eta2;            /* damping factors while dialling / waiting */
+    int32_t pause_on_eq;
+    int32_t reserved;
+} ms_relax_params;
+
+typedef struct ms_relax_report {
+    int32_t status;      /* 0 dialling, 1 waiting, 2 final equilibrium */
+    int32_t halted;
+    int32_t failed;      /* blow-up (see fail_step) */
+    int32_t num_eq;      /* equilibria reached */
+    int64_t eq_step;     /* clock step of the last equilibrium (-1 if none) */
+    int64_t fail_step;
+    double dial_factor;
+    double max_force_ratio;
+    double max_speed_ratio;
+} ms_relax_report;
+
+/* Arm the protocol for one shell (the current state is the start). */
+int ms_relax_setup(ms_ctx* ctx, int32_t shell, const ms_relax_params* p);
+/* Step all shells until every armed shell has halted or max_steps elapsed,
+ * looking at the device every poll_every steps.  reports: num_shells entries
+ * (may be NULL).  Returns MS_OK even if some shells blew up (see reports). */
+int ms_relax_run(ms_ctx* ctx, int64_t max_steps, int64_t poll_every, ms_relax_report* reports);
 /* Un-halt a shell (e.g. after the driver handled an equilibrium event). */
 int ms_resume_shell(ms_ctx* ctx, int32_t shell);
 

@@ -315,6 +315,8 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
         S.dial_last_eq = 1.0;
         S.dial_spacing = 0.0;
         S.dial_K = 1.0;
+        S.f_override = -1.0;
+        S.last_eq_step = -1;
         S.halt_step = -1;
         S.fail_step = -1;
         S.has_a0 = d.a0 ? 1 : 0;
@@ -530,6 +532,7 @@ extern "C" int ms_set_dial_clock(ms_ctx* c, int32_t s, double last_eq, double sp
     S.dial_K = (double)K;
     S.phase_steps = phase;
     S.dialling = dialling ? 1 : 0;
+    S.f_override = -1.0;
     return push_shells(c);
 }
 
@@ -537,8 +540,7 @@ extern "C" int ms_get_dial_factor(ms_ctx* c, int32_t s, double* f, int64_t* phas
     if (int r = shell_ok(c, s)) return r;
     if (int r = pull_shells(c)) return r;
     const ShellDev& S = c->hs[s];
-    double d = S.dial_last_eq + S.dial_spacing * ((double)S.phase_steps / S.dial_K);
-    if (d >= 1.0) d = 1.0;
+    const double d = shell_dial_factor(S);
     if (f) *f = d;
     if (phase) *phase = S.phase_steps;
     return MS_OK;
@@ -877,10 +879,104 @@ extern "C" int ms_step(ms_ctx* c, int64_t nsteps, int64_t equil_every, double to
     return status;
 }
 
+extern "C" int ms_relax_setup(ms_ctx* c, int32_t s, const ms_relax_params* p) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!p || p->steps_per_dial_phase <= 0 || !(p->dial_spacing > 0) || p->dial_spacing > 1.0 ||
+        p->equil_interval < 0 || !(p->tol_f > 0) || !(p->tol_v > 0))
+        return fail(c, MS_ERR_ARG, "ms_relax_setup: bad parameters");
+    if (int r = pull_shells(c)) return r;
+    ShellDev& S = c->hs[s];
+    S.auto_protocol = 1;
+    S.status = 0;
+    S.dialling = 1;
+    S.check_due = 0;
+    S.pause_on_eq = p->pause_on_eq ? 1 : 0;
+    S.num_eq = 0;
+    S.since_check = 0;
+    S.equil_interval = p->equil_interval;
+    S.eta1 = p->eta1;
+    S.eta2 = p->eta2;
+    S.eta = p->eta1;
+    S.tol_f = p->tol_f;
+    S.tol_v = p->tol_v;
+    S.last_eq_step = -1;
+    S.dial_K = (double)p->steps_per_dial_phase;
+    S.dial_spacing = p->dial_spacing;
+    S.dial_last_eq = p->dial_start;
+    // main.cpp:261: the phase counter of a restart resumes where the dial
+    // factor left it; the first step uses the start factor itself.
+    S.phase_steps = (long long)(std::fmod(p->dial_start, p->dial_spacing) * (double)p->steps_per_dial_phase);
+    S.f_override = p->dial_start;
+    S.f_targets = p->dial_start;  // step 0 always dials (main.cpp:334)
+    if (S.phase_steps >= p->steps_per_dial_phase) {  // main.cpp:293-304 at step 0
+        S.status = 1;
+        S.dialling = 0;
+        S.eta = S.eta2;
+    }
+    S.halted = 0;
+    S.failed = 0;
+    S.halt_step = -1;
+    S.fail_step = -1;
+    return push_shells(c);
+}
+
+extern "C" int ms_relax_run(ms_ctx* c, int64_t max_steps, int64_t poll_every, ms_relax_report* reports) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    if (max_steps < 0 || poll_every <= 0) return fail(c, MS_ERR_ARG, "ms_relax_run: bad step counts");
+    int64_t done = 0;
+    while (done < max_steps) {
+        if (int r = pull_shells(c)) return r;
+        bool running = false;
+        for (const auto& S : c->hs) running |= (S.auto_protocol && !S.halted);
+        if (!running) break;
+        const int64_t n = std::min<int64_t>(poll_every, max_steps - done);
+        const int r = ms_step(c, n, 0, 0.0, 0.0, 0, nullptr);
+        if (r != MS_OK && r != MS_ERR_HIGH_FORCE) return r;
+        done += n;
+    }
+    if (int r = pull_shells(c)) return r;
+    for (int s = 0; s < c->num_shells && reports; ++s) {
+        const ShellDev& S = c->hs[s];
+        ms_relax_report& R = reports[s];
+        R.status = S.status;
+        R.halted = S.halted;
+        R.failed = S.failed;
+        R.num_eq = S.num_eq;
+        R.eq_step = S.last_eq_step;
+        R.fail_step = S.fail_step;
+        R.dial_factor = shell_dial_factor(S);
+        R.max_force_ratio = S.max_f;
+        R.max_speed_ratio = S.max_v;
+    }
+    c->err.clear();
+    return MS_OK;
+}
+
 extern "C" int ms_resume_shell(ms_ctx* c, int32_t s) {
     if (int r = shell_ok(c, s)) return r;
     if (int r = pull_shells(c)) return r;
     ShellDev& S = c->hs[s];
+    if (S.auto_protocol && S.halted && !S.failed) {
+        if (S.status == 1 && S.last_eq_step == S.halt_step) {
+            // paused at an intermediate equilibrium: keep that step's update
+            // (its node launch already wrote it) and start the next dialling
+            // phase (main.cpp:483-485, 501-507)
+            S.par ^= 1;
+            S.f_override = S.dial_last_eq + S.dial_spacing * ((double)S.phase_steps / S.dial_K);
+            S.status = 0;
+            S.dialling = 1;
+            S.phase_steps = 0;
+            S.eta = S.eta1;
+            S.f_targets = shell_dial_factor(S);
+        } else if (S.status == 2) {
+            // re-open a final equilibrium (e.g. to finish in another precision)
+            S.status = 1;
+            S.dialling = 0;
+            S.eta = S.eta2;
+        }
+        S.since_check = 0;
+        S.check_due = 0;
+    }
     S.halted = 0;
     S.failed = 0;
     S.at_eq = 0;

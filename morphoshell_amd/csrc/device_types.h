@@ -50,8 +50,31 @@ struct ShellDev {
     int f_node;
     int v_node;
     int checked;
-    int pad_;
+    // Device-run relaxation protocol (main.cpp:269-507): status machine
+    // dialling -> waiting_for_equil -> equil_reached, checks every
+    // equil_interval steps while waiting, damping eta1 while dialling and
+    // eta2 while waiting.  auto_protocol = 0 leaves the status to the host.
+    int auto_protocol;
+    int status;            // 0 dialling, 1 waiting_for_equil, 2 equil_reached (final)
+    int check_due;         // the next step runs an equilibrium check
+    int pause_on_eq;       // halt at every equilibrium (host handles the event)
+    int num_eq;            // equilibria reached so far
+    long long since_check;
+    long long equil_interval;
+    double eta1, eta2;
+    double tol_f, tol_v;
+    long long last_eq_step;
+    double f_override;     // >= 0: dial factor of the next step (main.cpp:483-485 ordering)
+    double f_targets;      // factor abar/bbar were last dialled at (do_dialling runs only
+                           // while dialling or at step 0, main.cpp:334-353)
 };
+
+// Dial factor a step uses (main.cpp:286-288, 483-485).
+__host__ __device__ inline double shell_dial_factor(const ShellDev& s) {
+    double f = s.f_override >= 0.0 ? s.f_override
+                                   : s.dial_last_eq + s.dial_spacing * ((double)s.phase_steps / s.dial_K);
+    return f >= 1.0 ? 1.0 : f;
+}
 
 // A contiguous range of triangles or nodes of one shell, processed by one
 // block: {begin, end, shell, unused}.

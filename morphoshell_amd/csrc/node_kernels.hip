@@ -205,7 +205,8 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
         ShellDev* S = &k.shells[s];
         const int halted = S->halted;
         if (MODE == 0 && halted) continue;
-        if (CHECK) {
+        const bool proto_check = MODE == 0 && S->auto_protocol && S->check_due;
+        if (CHECK || proto_check) {
             double fm = -1.0, vm = -1.0;
             int fi = INT_MAX, vi = INT_MAX;
             const int c0 = S->node_chunk_begin, c1 = c0 + S->node_chunk_count;
@@ -227,7 +228,8 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
             S->f_node = fi - S->node_begin;
             S->v_node = vi - S->node_begin;
             S->checked = 1;
-            S->at_eq = (fr < k.fl.tol_f && vr < k.fl.tol_v) ? 1 : 0;
+            const double tf = CHECK ? k.fl.tol_f : S->tol_f, tv = CHECK ? k.fl.tol_v : S->tol_v;
+            S->at_eq = (fr < tf && vr < tv) ? 1 : 0;
         }
         if (MODE == 2) continue;
         if (MODE == 0) {
@@ -242,9 +244,56 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
                 S->halt_step = step;
                 continue;
             }
+            if (S->auto_protocol) {
+                // main.cpp:396-407: the verdict of this step's check
+                const double f_used = shell_dial_factor(*S);
+                S->f_override = -1.0;
+                bool eq_event = false;
+                if (proto_check) {
+                    S->since_check = 0;
+                    if (S->at_eq) {
+                        eq_event = true;
+                        S->last_eq_step = step;
+                        S->num_eq += 1;
+                    }
+                }
+                if (eq_event) {
+                    S->dial_last_eq = f_used;
+                    if (f_used >= 1.0 || S->pause_on_eq) {
+                        // keep the checked state: the input of this step
+                        S->status = f_used >= 1.0 ? 2 : 1;
+                        S->halted = 1;
+                        S->halt_step = step;
+                        S->check_due = 0;
+                        continue;
+                    }
+                    // main.cpp:483-485 computes the next step's factor before
+                    // main.cpp:501-507 restarts the phase counter
+                    S->f_override = f_used + S->dial_spacing * ((double)S->phase_steps / S->dial_K);
+                    S->status = 0;
+                    S->dialling = 1;
+                    S->phase_steps = 0;
+                    S->eta = S->eta1;
+                } else {
+                    if (S->status == 0) S->phase_steps += 1;
+                    if (S->status == 1) S->since_check += 1;
+                }
+                // main.cpp:293-304 at the start of the next step
+                if (S->status == 0 && S->phase_steps >= (long long)S->dial_K) {
+                    S->status = 1;
+                    S->dialling = 0;
+                    S->since_check = 0;
+                    S->eta = S->eta2;
+                }
+                S->check_due = (S->status == 1 && S->since_check > S->equil_interval) ? 1 : 0;
+                if (S->status == 0) S->f_targets = shell_dial_factor(*S);
+                S->par ^= 1;
+                continue;
+            }
         }
         if (MODE == 1 && k.b.bad_dof[s] != LLONG_MAX) continue;
         S->par ^= 1;
+        S->f_override = -1.0;
         if (S->dialling) S->phase_steps += 1;
     }
     __syncthreads();
@@ -260,6 +309,7 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
         const Chunk ch = k.chunks[ci];
         const ShellDev& sh = k.shells[ch.z];
         if (sh.halted) continue;  // block-uniform
+        const bool do_check = CHECK || sh.check_due;  // block-uniform
         const int n = ch.x + threadIdx.x;
         double fnorm = -1.0, vnorm = -1.0;
         int fidx = INT_MAX, vidx = INT_MAX;
@@ -282,7 +332,7 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
                 fz = damped(fz, sh.eta, m, vz, sh.damp_den);
             }
             if (k.nd.clamp[n]) fx = fy = fz = 0.0;
-            if (CHECK) {
+            if (do_check) {
                 fnorm = norm3(fx, fy, fz);
                 vnorm = norm3(vx, vy, vz);
                 fidx = vidx = n;
@@ -301,7 +351,7 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
             Vo[N + n] = vy;
             Vo[2 * N + n] = vz;
         }
-        if (CHECK) {
+        if (do_check) {
             double v2[2] = {fnorm, vnorm};
             int i2[2] = {fidx, vidx};
             block_argmax<2>(v2, i2);

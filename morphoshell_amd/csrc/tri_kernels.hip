@@ -40,9 +40,10 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
     return j * q + (j < r ? j : r) + local;
 }
 
+// Factor the programmed metric is evaluated at: under the device-run protocol
+// the last dialled one; otherwise the dial clock's.
 __device__ __forceinline__ double dial_factor(const ShellDev& s) {
-    double f = s.dial_last_eq + s.dial_spacing * ((double)s.phase_steps / s.dial_K);
-    return f >= 1.0 ? 1.0 : f;
+    return s.auto_protocol ? s.f_targets : shell_dial_factor(s);
 }
 
 // Programmed abar, bbar of triangle t at dial factor f (do_dialling.cpp:83-84

@@ -27,7 +27,8 @@ EXPORTED = [
     "ms_impose_constraints", "ms_advance_dynamics", "ms_is_at_equilibrium", "ms_calc_energies",
     "ms_calc_geometry", "ms_calc_energy_densities", "ms_get_forces", "ms_set_forces", "ms_step",
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
-    "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy",
+    "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_relax_setup",
+    "ms_relax_run",
 ]
 
 
@@ -97,6 +98,25 @@ class Tuning(C.Structure):
     ]
 
 
+class RelaxParams(C.Structure):
+    _fields_ = [
+        ("tol_f", C.c_double), ("tol_v", C.c_double), ("equil_interval", C.c_int64),
+        ("steps_per_dial_phase", C.c_int64), ("dial_spacing", C.c_double), ("dial_start", C.c_double),
+        ("eta1", C.c_double), ("eta2", C.c_double), ("pause_on_eq", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class RelaxReport(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("halted", C.c_int32), ("failed", C.c_int32), ("num_eq", C.c_int32),
+        ("eq_step", C.c_int64), ("fail_step", C.c_int64), ("dial_factor", C.c_double),
+        ("max_force_ratio", C.c_double), ("max_speed_ratio", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -151,6 +171,8 @@ def lib() -> C.CDLL:
         "ms_device_bytes": (C.c_int64, [P]),
         "ms_synchronize": (C.c_int, [P]),
         "ms_calibration_copy": (C.c_int, [C.c_int64, C.c_int32, _dp]),
+        "ms_relax_setup": (C.c_int, [P, C.c_int32, C.POINTER(RelaxParams)]),
+        "ms_relax_run": (C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(RelaxReport)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

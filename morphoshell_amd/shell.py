@@ -458,6 +458,30 @@ class Context:
     def resume(self, shell: int) -> None:
         self._ck(self.L.ms_resume_shell(self.h, shell))
 
+    # -- device-run relaxation protocol (main.cpp:269-507)
+    def relax_setup(self, shell: int, steps_per_dial_phase: int | None = None, dial_spacing: float | None = None,
+                    dial_start: float = 0.0, equil_interval: int | None = None, tol_f: float | None = None,
+                    tol_v: float | None = None, pause_on_eq: bool = False) -> None:
+        sh = self.shells[shell]
+        st, sc = sh.settings, sh.scales
+        p = nat.RelaxParams()
+        p.tol_f = st.force_to_char_force_ratio_equil_threshold if tol_f is None else tol_f
+        p.tol_v = st.speed_to_char_speed_ratio_equil_threshold if tol_v is None else tol_v
+        p.equil_interval = sc["timesteps_between_equil_checks"] if equil_interval is None else equil_interval
+        p.steps_per_dial_phase = sc["timesteps_per_dial_phase"] if steps_per_dial_phase is None else \
+            steps_per_dial_phase
+        p.dial_spacing = st.dial_factors_to_hold_at_spacing if dial_spacing is None else dial_spacing
+        p.dial_start = dial_start
+        p.eta1 = scales_for(sh, st.damping_prefactor_1)["damping_factor"]
+        p.eta2 = scales_for(sh, st.damping_prefactor_2)["damping_factor"]
+        p.pause_on_eq = 1 if pause_on_eq else 0
+        self._ck(self.L.ms_relax_setup(self.h, shell, C.byref(p)))
+
+    def relax_run(self, max_steps: int, poll_every: int = 2000) -> list[dict]:
+        reps = (nat.RelaxReport * len(self.shells))()
+        self._ck(self.L.ms_relax_run(self.h, int(max_steps), int(poll_every), reps))
+        return [r.as_dict() for r in reps]
+
     @property
     def steps_taken(self) -> int:
         return int(self.L.ms_steps_taken(self.h))

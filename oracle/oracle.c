@@ -639,3 +639,7 @@ EXPORT void orc_tri_energy_q(const double* q, const double* abar, const double* 
     out[1] = e.b;
     if (g) dE_dq_one(&s, 0, q, g);
 }
+
+/* Change the damping factor (the reference recomputes it at phase switches,
+ * main.cpp:301, 505). */
+EXPORT void orc_set_eta(orc_sys* s, double eta) { s->eta = eta; }

@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
     L.orc_dial.argtypes = [C.c_int32, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp,
                            _dp]
     L.orc_tri_energy_q.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
+    L.orc_set_eta.argtypes = [P, C.c_double]
     L.orc_num_threads.restype = C.c_int
     L.orc_set_num_threads.argtypes = [C.c_int]
     _lib = L
@@ -210,3 +211,60 @@ class System:
         f = np.zeros(3 * self.N)
         r = lib().orc_run_steps(self.h, _d(dofs), _d(vel), _d(f), nsteps)
         return int(r)
+
+
+def run_protocol(sysm: System, dial, x, v, K, spacing, start, interval, eta1, eta2, tol_f, tol_v, max_steps):
+    """The reference's dynamics-loop state machine (main.cpp:269-507) over the
+    oracle's per-step calls, for one shell, stopping at the first equilibrium
+    whose dial factor is 1 (the reference's own loop never stops there,
+    main.cpp:493).  ``dial(f)`` returns (abar, bbar) 3T SoA at dial factor f.
+
+    Returns dict(steps, eq_steps, status, dial_factor) and updates x, v in place
+    (x, v end as the checked state of the final equilibrium, or after
+    max_steps).
+    """
+    import math
+
+    DIAL, WAIT, EQ = 0, 1, 2
+    status = DIAL
+    step = 0
+    phase = int(math.fmod(start, spacing) * K)
+    last_eq = start
+    f = start
+    since_check = -123456
+    eq_steps = []
+    lib().orc_set_eta(sysm.h, eta1)
+    while step < max_steps:
+        if f >= 1.0:
+            f = 1.0
+        if status == DIAL and phase >= K:
+            status = WAIT
+            since_check = 0
+            lib().orc_set_eta(sysm.h, eta2)
+        if status == DIAL or step == 0:
+            ab, bb = dial(f)
+            sysm.set_targets(ab, bb)
+        F = sysm.full_forces(x, v)
+        if status == WAIT and since_check > interval:
+            fr, _, vr, _ = sysm.equilibrium(F, v)
+            if fr < tol_f and vr < tol_v:
+                status = EQ
+                last_eq = f
+                eq_steps.append(step)
+            since_check = 0
+        if status == EQ and f >= 1.0:
+            return {"steps": step, "eq_steps": eq_steps, "status": 2, "dial_factor": f}
+        d, _ = sysm.advance(x, v, F)
+        if d >= 0:
+            return {"steps": step, "eq_steps": eq_steps, "status": -1, "dial_factor": f, "bad_dof": d}
+        step += 1
+        if status == DIAL:
+            phase += 1
+        f = last_eq + spacing * (phase / K)
+        if status == WAIT:
+            since_check += 1
+        if status == EQ:
+            status = DIAL
+            phase = 0
+            lib().orc_set_eta(sysm.h, eta1)
+    return {"steps": step, "eq_steps": eq_steps, "status": status, "dial_factor": f}
