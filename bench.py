@@ -1,24 +1,29 @@
 #!/usr/bin/env python3
 """Per-step throughput of the shell force-and-dynamics path on one MI355X.
 
-Workload (default): the large single shell -- unit disc, jittered triangular
-lattice at spacing 0.002 (seed 0) + boundary ring, Delaunay (~1.81M
-triangles, ~0.91M nodes); isotropic programmed metric abar = lambda^2 I with
+Default workload (the metric's line): the large single shell -- unit disc,
+jittered triangular lattice at spacing 0.002 (seed 0) + boundary ring,
+Delaunay (~1.81M triangles, ~0.91M nodes); abar = lambda^2 I with
 lambda = 1 + 0.1 g(x), g a unit-variance Gaussian random field of correlation
 length 0.2 from 64 seeded Fourier modes; bbar = 0; h = 0.005, nu = 0.5,
 mu = rho = 1; flat start with Gaussian z-noise (sigma 1e-4).  Synthetic data.
 
 A step is one pass of the path over the shell: elastic forces of every
 triangle, deterministic per-node gather, damping, clamp, blow-up check and
-symplectic-Euler update (two launches, replayed from a HIP graph).  ``value``
-is triangle force-evaluations per second of the FP64 path (inputs resident in
-HBM); the FP32-force/FP64-state mode is reported beside it.  The roofline of
-the dominant (triangle) kernel uses its algorithmic bytes: the 288 B (FP64) /
-180 B (mixed) per-triangle record plus 24 B per node of positions.  The CPU
-baseline is the FP64 OpenMP oracle on the host cores, on a bounded number of
-steps of the same shell.
+symplectic-Euler update, replayed from a HIP graph.  ``value`` is triangle
+force-evaluations per second of the FP64 path (inputs resident in HBM); the
+FP32-force/FP64-state mode is reported beside it.  The roofline is that of
+the dominant kernel with its algorithmic bytes (see ``kern_bytes`` below).
+The CPU baseline is the FP64 OpenMP oracle on the host cores, on a bounded
+number of steps of the same shell.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (one JSON line each):
+  --workload batch   ragged batch of 512 discs/squares, 2k-100k triangles,
+                     mixed mode, autotuned launch shapes
+  --workload relax   time-to-equilibrium of the small defect disc: FP64 vs
+                     mixed-then-FP64, beside the oracle on the host
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload single|batch|relax]
 """
 from __future__ import annotations
 
@@ -40,46 +45,39 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_shell(spacing: float, seed: int = 0):
-    from morphoshell_amd.meshing import disc_mesh, grf, isotropic_abar_info
-    from morphoshell_amd.shell import Settings, prepare_shell
-
-    t0 = time.time()
-    m = disc_mesh(spacing, seed=seed)
-    c = m.centroids()
-    m.abar_info = isotropic_abar_info(1.0 + 0.1 * grf(c[:, :2], 0.2, n_modes=64, seed=seed + 1))
-    m.bbar_info = np.zeros((m.num_tris, 3))
-    m.fill_defaults(h=0.005, mu=1.0)
-    t1 = time.time()
-    sh = prepare_shell(m, Settings(def_poisson_ratio=0.5, ref_density=1.0))
-    t2 = time.time()
-    return sh, t1 - t0, t2 - t1
-
-
-def initial_state(sh, seed: int = 0):
-    rng = np.random.default_rng(seed)
-    x = sh.ref_pos.copy()
-    x[2 * sh.N:] += 1e-4 * rng.standard_normal(sh.N)
-    return x, np.zeros_like(x)
-
-
-def time_steps(ctx, x, v, steps: int, warmup: int) -> float:
-    ctx.set_state(0, x, v)
-    ctx.step(warmup)
+def time_steps(ctx, states, steps: int, warmup: int, strict: bool = True) -> float:
+    """Seconds for ``steps`` fused steps (after ``warmup``).  With strict=False a
+    shell that blows up is halted and the others keep stepping."""
+    for i, (x, v) in enumerate(states):
+        ctx.set_state(i, x, v)
+        ctx.resume(i)
+    ctx.step(warmup, raise_on_blowup=strict)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(steps)
+    ctx.step(steps, raise_on_blowup=strict)
     ctx.synchronize()
     return time.perf_counter() - t0
 
 
-def kernel_times(ctx, x, v, steps: int) -> np.ndarray:
-    ctx.set_state(0, x, v)
+def kernel_times(ctx, states, steps: int, strict: bool = True) -> np.ndarray:
+    for i, (x, v) in enumerate(states):
+        ctx.set_state(i, x, v)
+        ctx.resume(i)
     ctx.enable_timing(True)
-    ctx.step(steps)
+    ctx.step(steps, raise_on_blowup=strict)
     t = ctx.last_step_timing()
     ctx.enable_timing(False)
     return t
+
+
+def kernel_bytes(T: int, N: int, mixed: bool) -> list[int]:
+    """Algorithmic bytes per launch.  Triangle kernel: the per-triangle record
+    (6 patch ids 24 B + G, H 192 B FP64 / 96 B FP32 + abar/bbar dial info 48 B
+    + h, mu, area 24 B / 12 B) and each node position once (24 B).  Node
+    kernel: x, v read and written (96 B) and the mass (8 B).  The per-slot
+    contributions passed between the two launches are intermediates."""
+    rec = 180 if mixed else 288
+    return [T * rec + N * 24, N * 104]
 
 
 def cpu_baseline(sh, x, v, budget_s: float) -> dict:
@@ -105,44 +103,33 @@ def cpu_baseline(sh, x, v, budget_s: float) -> dict:
     }
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--spacing", type=float, default=0.002)
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--calibrate", action="store_true", help="also run the copy kernel used to scale counters")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (each shell, or batch of shells, lives on one device); "
-              "multi-GPU runs are not part of it.", file=sys.stderr)
-        return 2
+# ------------------------------------------------------------------ single
 
-    from morphoshell_amd import native
+def bench_single(args) -> dict:
+    from morphoshell_amd import native, workloads
     from morphoshell_amd.shell import Context
 
     log(f"building shell, spacing {args.spacing}")
-    sh, t_mesh, t_prep = make_shell(args.spacing)
+    t0 = time.time()
+    sh = workloads.grf_disc(args.spacing)
+    t_setup = time.time() - t0
     T, N = sh.T, sh.N
-    log(f"T={T} N={N} mesh {t_mesh:.1f}s set-up {t_prep:.1f}s")
-    x, v = initial_state(sh)
-
+    log(f"T={T} N={N} mesh + set-up {t_setup:.1f}s")
+    x, v = workloads.flat_noisy_state(sh)
+    states = [(x, v)]
     res = {}
     for prec in ("fp64", "mixed"):
         with Context(sh, precision=prec) as ctx:
-            secs = time_steps(ctx, x, v, args.steps, args.warmup)
-            kt = kernel_times(ctx, x, v, min(args.steps, 50))
+            secs = time_steps(ctx, states, args.steps, args.warmup)
+            kt = kernel_times(ctx, states, min(args.steps, 50))
             res[prec] = {"secs": secs, "kt": kt}
             log(f"{prec}: {args.steps} steps {secs * 1e3:.2f} ms -> {T * args.steps / secs / 1e9:.3f} G tri-evals/s;"
                 f" kernel ms tri {kt[0]:.4f} node {kt[1]:.4f}")
-            if prec == "fp64" and args.calibrate:
-                cal_ms = np.zeros(1)
-                native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
-                res["copy_GBs"] = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
-                log(f"copy kernel 1 GiB: {cal_ms[0]:.3f} ms = {res['copy_GBs']:.0f} GB/s")
+    if args.calibrate:
+        cal_ms = np.zeros(1)
+        native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
+        res["copy_GBs"] = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
+        log(f"copy kernel 1 GiB: {cal_ms[0]:.3f} ms = {res['copy_GBs']:.0f} GB/s")
 
     parity = {}
     if not args.no_parity:
@@ -172,16 +159,12 @@ def main() -> int:
     ms_step = r64["secs"] / args.steps * 1e3
     kt = r64["kt"]
     dominant = 0 if kt[0] >= kt[1] else 1
-    # algorithmic bytes per launch: the triangle kernel streams the 288 B
-    # record of every triangle and reads each node's position once; the node
-    # kernel reads x, v (48 B) and the mass (8 B) and writes x, v (48 B).
-    # The per-slot contributions passed between them are intermediates.
-    kern_bytes = [T * 288 + N * 24, N * 104]
-    kern_names = ["tri_fp64_kernel (per-triangle forces)", "node_step_kernel (gather + dynamics)"]
-    achieved = kern_bytes[dominant] / (kt[dominant] * 1e-3) / 1e9
+    kb = kernel_bytes(T, N, False)
+    names = ["tri_fp64_kernel (per-triangle forces)", "node_step_kernel (gather + dynamics)"]
+    achieved = kb[dominant] / (kt[dominant] * 1e-3) / 1e9
     step_bytes = T * 288 + N * 104
     rm = res["mixed"]
-    mixed_value = T * args.steps / rm["secs"]
+    kbm = kernel_bytes(T, N, True)
     out = {
         "metric": "triangle force-evals/s",
         "value": value,
@@ -203,14 +186,14 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": kern_names[dominant],
+            "kernel": names[dominant],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "alg_bytes_per_launch": kern_bytes[dominant],
+            "alg_bytes_per_launch": kb[dominant],
             "avg_launch_ms": kt[dominant],
         },
         "step_roofline": {
@@ -220,19 +203,173 @@ def main() -> int:
             "kernel_ms": {"tri": kt[0], "node": kt[1]},
         },
         "mixed": {
-            "value": mixed_value,
+            "value": T * args.steps / rm["secs"],
             "ms_per_step": rm["secs"] / args.steps * 1e3,
             "dtype": "fp32 forces / f64 state",
             "kernel_ms": {"tri": rm["kt"][0], "node": rm["kt"][1]},
-            "tri_achieved_GBs": (T * 180 + N * 24) / (rm["kt"][0] * 1e-3) / 1e9,
+            "tri_achieved_GBs": kbm[0] / (rm["kt"][0] * 1e-3) / 1e9,
+            "step_alg_GBs": (T * 180 + N * 104) / (rm["secs"] / args.steps) / 1e9,
         },
         "parity": parity,
         "cpu_baseline": cpu,
-        "setup_s": {"mesh": t_mesh, "host_precompute": t_prep},
+        "setup_s": t_setup,
     }
     if "copy_GBs" in res:
         out["copy_kernel_GBs"] = res["copy_GBs"]
-    print(json.dumps(out), flush=True)
+    return out
+
+
+# ------------------------------------------------------------------- batch
+
+def bench_batch(args) -> dict:
+    from morphoshell_amd import autotune, workloads
+    from morphoshell_amd.shell import Context
+
+    t0 = time.time()
+    shells = workloads.ragged_batch(args.shells, precision="mixed")
+    t_setup = time.time() - t0
+    T = sum(s.T for s in shells)
+    N = sum(s.N for s in shells)
+    log(f"batch of {len(shells)} shells: T={T} N={N}, set-up {t_setup:.1f}s")
+    states = [workloads.flat_noisy_state(s, seed=i) for i, s in enumerate(shells)]
+    with Context(shells, precision="mixed") as ctx:
+        for i, (x, v) in enumerate(states):
+            ctx.set_state(i, x, v)
+        tuned = autotune.tune(ctx, steps=20, use_cache=False, verbose=args.verbose)
+        log(f"autotuned: {tuned}")
+        secs = time_steps(ctx, states, args.steps, args.warmup, strict=False)
+        reps = ctx.step(0, raise_on_blowup=False)
+        blown = [i for i, r in enumerate(reps) if r["status"] != 0]
+        T_live = T - sum(shells[i].T for i in blown)
+        kt = kernel_times(ctx, states, min(args.steps, 30), strict=False)
+        # relaxation of every shell within a bounded number of steps
+        for i, (x, v) in enumerate(states):
+            ctx.set_state(i, x, v)
+            ctx.relax_setup(i, steps_per_dial_phase=1000, dial_spacing=1.0, dial_start=0.0, equil_interval=200,
+                            tol_f=1e-6, tol_v=1e-5)
+        t1 = time.perf_counter()
+        reps = ctx.relax_run(args.relax_steps, poll_every=2000)
+        t_relax = time.perf_counter() - t1
+    done = sum(1 for r in reps if r["status"] == 2)
+    failed = sum(1 for r in reps if r["failed"])
+    kb = kernel_bytes(T, N, True)
+    return {
+        "metric": "triangle force-evals/s (ragged batch)",
+        "value": T_live * args.steps / secs,
+        "shells_blown_up_in_timing": len(blown),
+        "unit": "triangle force-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "ms_per_step": secs / args.steps * 1e3,
+        "dtype": "fp32 forces / f64 state",
+        "config": {"workload": f"ragged batch of {len(shells)} discs/squares, triangles log-uniform in [2k, 100k], "
+                               "per-shell GRF lambda, bbar = kappa I, h in [0.002, 0.02]",
+                   "num_shells": len(shells), "num_tris": T, "num_nodes": N},
+        "autotune": tuned,
+        "kernel_ms": {"tri": kt[0], "node": kt[1]},
+        "tri_alg_GBs": kb[0] / (kt[0] * 1e-3) / 1e9,
+        "step_alg_GBs": (T * 180 + N * 104) / (secs / args.steps) / 1e9,
+        "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
+                  "shells_failed": failed, "shells_per_s": done / t_relax if t_relax > 0 else 0.0,
+                  "tolerances": {"force": 1e-6, "speed": 1e-5}},
+        "setup_s": t_setup,
+    }
+
+
+# ------------------------------------------------------------------- relax
+
+def bench_relax(args) -> dict:
+    from morphoshell_amd import workloads
+    from morphoshell_amd.shell import Context
+    from oracle import oracle as orc
+
+    sh = workloads.small_defect_disc()
+    x0, v0 = workloads.flat_noisy_state(sh)
+    st, sc = sh.settings, sh.scales
+    K = st.timesteps_per_dial_phase
+    interval = max(1, int(round(st.time_between_equil_checks_prefactor * K)))
+    tol_f, tol_v = st.force_to_char_force_ratio_equil_threshold, st.speed_to_char_speed_ratio_equil_threshold
+    log(f"small defect disc T={sh.T}, dt={sc['timestep']:.3e}, K={K}, check every {interval}")
+
+    def arm(ctx):
+        ctx.set_state(0, x0, v0)
+        ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=interval,
+                        tol_f=tol_f, tol_v=tol_v)
+
+    out = {"metric": "time-to-equilibrium", "unit": "s", "higher_is_better": False,
+           "config": {"workload": f"small defect disc, {sh.T} triangles, LCE lambda 1->0.9 over {K} steps, "
+                                  f"h=0.01, force tol {tol_f}, speed tol {tol_v}"}}
+    with Context(sh, precision="fp64") as ctx:
+        arm(ctx)
+        t0 = time.perf_counter()
+        r64 = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
+        t64 = time.perf_counter() - t0
+        x64, v64 = ctx.get_state(0)
+        E64 = ctx.calc_energies(0)
+    log(f"fp64: {r64} in {t64:.2f}s")
+    with Context(sh, precision="mixed") as ctx:
+        arm(ctx)
+        t0 = time.perf_counter()
+        rm = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
+        steps_mixed = ctx.steps_taken
+        ctx.set_precision("fp64")
+        ctx.resume(0)
+        rf = ctx.relax_run(args.relax_steps, poll_every=5000)[0]
+        tm = time.perf_counter() - t0
+        xm, _ = ctx.get_state(0)
+        Em = ctx.calc_energies(0)
+        steps_total = ctx.steps_taken
+    log(f"mixed: {rm}, then fp64 {rf}, {tm:.2f}s")
+    # the CPU oracle runs the same protocol for a bounded time; the rest is
+    # extrapolated at its measured per-step rate
+    sysm = orc.System.from_shell(sh)
+    xx, vv = x0.copy(), v0.copy()
+    budget = args.cpu_budget
+    steps_cpu = 2000
+    t0 = time.perf_counter()
+    orc.run_protocol(sysm, lambda f: orc.dial(sh.T, 3, False, f, sh.mesh.abar_info, None), xx, vv, K, 1.0, 0.0,
+                     interval, sc["damping_factor"], sc["damping_factor"], tol_f, tol_v, steps_cpu)
+    cpu_per_step = (time.perf_counter() - t0) / steps_cpu
+    n64 = (r64["eq_step"] + 1) if r64["status"] == 2 else args.relax_steps
+    out.update({
+        "fp64": {"seconds": t64, "steps": n64, "status": r64["status"], "max_force_ratio": r64["max_force_ratio"],
+                 "energy": (E64[0] + E64[1])},
+        "mixed_then_fp64": {"seconds": tm, "steps_mixed": steps_mixed, "steps_total": steps_total,
+                            "status": rf["status"], "max_force_ratio": rf["max_force_ratio"],
+                            "energy": (Em[0] + Em[1]),
+                            "energy_rel_diff_vs_fp64": abs((Em[0] + Em[1]) - (E64[0] + E64[1])) /
+                            abs(E64[0] + E64[1]),
+                            "max_position_diff_vs_fp64": float(np.abs(xm - x64).max())},
+        "speedup_mixed_vs_fp64": t64 / tm if tm > 0 else None,
+        "cpu_baseline": {"seconds_extrapolated": cpu_per_step * n64, "per_step_s": cpu_per_step,
+                         "cores": orc.lib().orc_num_threads(), "kind": "port",
+                         "sample": f"{steps_cpu} steps of the same protocol, extrapolated to {n64} steps"},
+    })
+    del budget
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax"])
+    ap.add_argument("--spacing", type=float, default=0.002)
+    ap.add_argument("--shells", type=int, default=512)
+    ap.add_argument("--relax-steps", type=int, default=20000)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate", action="store_true", help="also run the copy kernel used to scale counters")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (a shell, or a batch of shells, lives on one device); "
+              "runs on several GPUs are not part of it.", file=sys.stderr)
+        return 2
+    fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
     return 0
 
 

@@ -292,7 +292,8 @@ typedef struct ms_relax_report {
     int32_t halted;
     int32_t failed;      /* blow-up (see fail_step) */
     int32_t num_eq;      /* equilibria reached */
-    int64_t eq_step;     /* clock step of the last equilibrium (-1 if none) */
+    int64_t eq_step;     /* the shell's own step count (since ms_relax_setup) at its
+                            last equilibrium check that passed; -1 if none */
     int64_t fail_step;
     double dial_factor;
     double max_force_ratio;

@@ -1,0 +1,106 @@
+"""Launch-shape autotuner of the per-step kernels.
+
+Searches triangles-per-block, nodes-per-block, persistent-grid occupancy and
+steps per captured HIP graph for a context, timing each candidate with the
+library's own HIP events, and caches the winner by workload signature (total
+triangles and nodes bucketed to powers of two, shell count bucket, precision,
+device) in a JSON file (``$MS_TUNE_CACHE`` or
+``~/.cache/morphoshell_amd/tuning.json``).  Results do not depend on the
+launch shape (the gather order is fixed by the CSR), so tuning never changes
+numbers, only time.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import os
+import time
+
+from .shell import Context
+
+DEFAULT_SPACE = {
+    "tri_block": (64, 128, 256),
+    "node_block": (128, 256),
+    "grid_per_cu": (0, 4, 8),
+    "graph_steps": (16,),
+}
+
+
+def _cache_path() -> str:
+    p = os.environ.get("MS_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "morphoshell_amd", "tuning.json")
+
+
+def signature(ctx: Context) -> str:
+    T = sum(s.T for s in ctx.shells)
+    N = sum(s.N for s in ctx.shells)
+    S = len(ctx.shells)
+    b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
+    prec = "mixed" if ctx.precision == 1 else "fp64"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}"
+
+
+def _load() -> dict:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d: dict) -> None:
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        with open(p + ".tmp", "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(p + ".tmp", p)
+    except OSError:
+        pass
+
+
+def time_config(ctx: Context, steps: int) -> float:
+    """Seconds per step of the fused path under the context's current tuning."""
+    ctx.step(2, raise_on_blowup=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps, raise_on_blowup=False)
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bool = True,
+         verbose: bool = False) -> dict:
+    """Pick and apply the fastest launch shape for ``ctx``.
+
+    The context's state is advanced by the timing steps; callers set the
+    state they want afterwards.  Returns the chosen configuration with its
+    time per step.
+    """
+    sig = signature(ctx)
+    cache = _load() if use_cache else {}
+    if sig in cache:
+        best = cache[sig]
+        ctx.set_tuning(tri_block=best["tri_block"], node_block=best["node_block"],
+                       tri_grid_per_cu=best["grid_per_cu"], node_grid_per_cu=best["grid_per_cu"],
+                       graph_steps=best["graph_steps"])
+        return dict(best, cached=True)
+    sp = space or DEFAULT_SPACE
+    results = []
+    for tb, nb, g, gs in itertools.product(sp["tri_block"], sp["node_block"], sp["grid_per_cu"], sp["graph_steps"]):
+        ctx.set_tuning(tri_block=tb, node_block=nb, tri_grid_per_cu=g, node_grid_per_cu=g, graph_steps=gs)
+        t = time_config(ctx, steps)
+        results.append((t, {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs}))
+        if verbose:
+            print(f"[autotune] {sig} tri {tb} node {nb} grid/cu {g} graph {gs}: {t * 1e3:.4f} ms/step")
+    t, best = min(results, key=lambda r: r[0])
+    best = dict(best, ms_per_step=t * 1e3, signature=sig, candidates=len(results))
+    ctx.set_tuning(tri_block=best["tri_block"], node_block=best["node_block"], tri_grid_per_cu=best["grid_per_cu"],
+                   node_grid_per_cu=best["grid_per_cu"], graph_steps=best["graph_steps"])
+    if use_cache:
+        cache[sig] = best
+        _store(cache)
+    return dict(best, cached=False)

@@ -900,6 +900,7 @@ extern "C" int ms_relax_setup(ms_ctx* c, int32_t s, const ms_relax_params* p) {
     S.tol_f = p->tol_f;
     S.tol_v = p->tol_v;
     S.last_eq_step = -1;
+    S.own_steps = 0;
     S.dial_K = (double)p->steps_per_dial_phase;
     S.dial_spacing = p->dial_spacing;
     S.dial_last_eq = p->dial_start;
@@ -962,6 +963,7 @@ extern "C" int ms_resume_shell(ms_ctx* c, int32_t s) {
             // (its node launch already wrote it) and start the next dialling
             // phase (main.cpp:483-485, 501-507)
             S.par ^= 1;
+            S.own_steps += 1;
             S.f_override = S.dial_last_eq + S.dial_spacing * ((double)S.phase_steps / S.dial_K);
             S.status = 0;
             S.dialling = 1;

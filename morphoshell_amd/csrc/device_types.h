@@ -67,6 +67,7 @@ struct ShellDev {
     double f_override;     // >= 0: dial factor of the next step (main.cpp:483-485 ordering)
     double f_targets;      // factor abar/bbar were last dialled at (do_dialling runs only
                            // while dialling or at step 0, main.cpp:334-353)
+    long long own_steps;   // steps this shell has taken (halted shells do not count)
 };
 
 // Dial factor a step uses (main.cpp:286-288, 483-485).

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
                     S->since_check = 0;
                     if (S->at_eq) {
                         eq_event = true;
-                        S->last_eq_step = step;
+                        S->last_eq_step = S->own_steps;  // in the shell's own step count
                         S->num_eq += 1;
                     }
                 }
@@ -288,11 +288,13 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
                 S->check_due = (S->status == 1 && S->since_check > S->equil_interval) ? 1 : 0;
                 if (S->status == 0) S->f_targets = shell_dial_factor(*S);
                 S->par ^= 1;
+                S->own_steps += 1;
                 continue;
             }
         }
         if (MODE == 1 && k.b.bad_dof[s] != LLONG_MAX) continue;
         S->par ^= 1;
+        S->own_steps += 1;
         S->f_override = -1.0;
         if (S->dialling) S->phase_steps += 1;
     }
