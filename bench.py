@@ -291,10 +291,15 @@ def bench_relax(args) -> dict:
     tol_f, tol_v = st.force_to_char_force_ratio_equil_threshold, st.speed_to_char_speed_ratio_equil_threshold
     log(f"small defect disc T={sh.T}, dt={sc['timestep']:.3e}, K={K}, check every {interval}")
 
-    def arm(ctx):
+    # mixed-mode forces carry an FP32 error of ~1e-6..1e-5 F_char near
+    # equilibrium, so the mixed phase stops at a looser residual and FP64
+    # finishes to the real tolerances
+    mixed_tol = 1e-4
+
+    def arm(ctx, tf=tol_f, tv=tol_v):
         ctx.set_state(0, x0, v0)
         ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=interval,
-                        tol_f=tol_f, tol_v=tol_v)
+                        tol_f=tf, tol_v=tv)
 
     out = {"metric": "time-to-equilibrium", "unit": "s", "higher_is_better": False,
            "config": {"workload": f"small defect disc, {sh.T} triangles, LCE lambda 1->0.9 over {K} steps, "
@@ -308,12 +313,14 @@ def bench_relax(args) -> dict:
         E64 = ctx.calc_energies(0)
     log(f"fp64: {r64} in {t64:.2f}s")
     with Context(sh, precision="mixed") as ctx:
-        arm(ctx)
+        arm(ctx, mixed_tol, mixed_tol)
         t0 = time.perf_counter()
         rm = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
         steps_mixed = ctx.steps_taken
         ctx.set_precision("fp64")
-        ctx.resume(0)
+        # continue from the mixed end state, already fully dialled
+        ctx.relax_setup(0, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=interval,
+                        tol_f=tol_f, tol_v=tol_v)
         rf = ctx.relax_run(args.relax_steps, poll_every=5000)[0]
         tm = time.perf_counter() - t0
         xm, _ = ctx.get_state(0)

@@ -12,6 +12,7 @@ numbers, only time.
 from __future__ import annotations
 
 import itertools
+import sys
 import json
 import math
 import os
@@ -95,7 +96,7 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
         t = time_config(ctx, steps)
         results.append((t, {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs}))
         if verbose:
-            print(f"[autotune] {sig} tri {tb} node {nb} grid/cu {g} graph {gs}: {t * 1e3:.4f} ms/step")
+            print(f"[autotune] {sig} tri {tb} node {nb} grid/cu {g} graph {gs}: {t * 1e3:.4f} ms/step", file=sys.stderr)
     t, best = min(results, key=lambda r: r[0])
     best = dict(best, ms_per_step=t * 1e3, signature=sig, candidates=len(results))
     ctx.set_tuning(tri_block=best["tri_block"], node_block=best["node_block"], tri_grid_per_cu=best["grid_per_cu"],

@@ -1,0 +1,335 @@
+// Kernels of the mesh message-passing network (learned warm start).
+//
+// GEMMs run on the matrix cores: v_mfma_f32_16x16x16_bf16 with operands
+// rounded to bfloat16 (round-to-nearest-even) as they are staged into LDS,
+// FP32 accumulation, FP32 activations and master weights in memory.  Every
+// reduction (split-K GEMM partials, bias gradients, edge aggregation, loss)
+// runs in a fixed order, so training is bitwise repeatable.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "nn_kernels.h"
+
+namespace ms {
+namespace nn {
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16 bits, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ short f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+// C[z] = A[:, kz] @ B[kz, :] for the K-range of slice z (or the whole K when
+// ksplit == 1), with A(i,k) = A[i*sa0 + k*sa1], B(k,j) = B[k*sb0 + j*sb1],
+// C(i,j) = C[z*M*N + i*ldc + j].  Epilogue: + bias[j] (slice 0 only) and
+// optional accumulation into C.  256 threads: wave w owns rows 16w..16w+15
+// of the 64x64 tile and four 16x16 MFMA accumulators along N.
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    __shared__ short As[BM][BK + 4];
+    __shared__ short Bs[BK][BN + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int z = blockIdx.z;
+    const int kbeg = z * g.kchunk;
+    const int kend = min(g.K, kbeg + g.kchunk);
+    v4f acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        // stage A (64 x 16) and B (16 x 64): 4 elements per thread each
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int idx = tid + r * 256;
+            const int ai = idx / BK, ak = idx % BK;
+            const int gi = m0 + ai, gk = k0 + ak;
+            float av = 0.f;
+            if (gi < g.M && gk < kend) av = g.A[(size_t)gi * g.sa0 + (size_t)gk * g.sa1];
+            As[ai][ak] = f2bf(av);
+            const int bk = idx / BN, bj = idx % BN;
+            const int hk = k0 + bk, gj = n0 + bj;
+            float bv = 0.f;
+            if (hk < kend && gj < g.N) bv = g.B[(size_t)hk * g.sb0 + (size_t)gj * g.sb1];
+            Bs[bk][bj] = f2bf(bv);
+        }
+        __syncthreads();
+        const int row = wave * 16 + (lane & 15);
+        const int kq = (lane >> 4) * 4;
+        v4s a;
+        a.x = As[row][kq];
+        a.y = As[row][kq + 1];
+        a.z = As[row][kq + 2];
+        a.w = As[row][kq + 3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int col = t * 16 + (lane & 15);
+            v4s b;
+            b.x = Bs[kq][col];
+            b.y = Bs[kq + 1][col];
+            b.z = Bs[kq + 2][col];
+            b.w = Bs[kq + 3][col];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout: lane holds column (lane & 15) of rows 4*(lane >> 4) + r
+    float* C = g.C + (size_t)z * g.M * g.ldc;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int col = n0 + t * 16 + (lane & 15);
+        if (col >= g.N) continue;
+        const float bias = (g.bias && z == 0) ? g.bias[col] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int rowg = m0 + wave * 16 + (lane >> 4) * 4 + r;
+            if (rowg >= g.M) continue;
+            float v = acc[t][r] + bias;
+            float* dst = C + (size_t)rowg * g.ldc + col;
+            *dst = g.accumulate ? *dst + v : v;
+        }
+    }
+}
+
+// out[i] = sum_z part[z][i] (+ out[i] if accumulate), fixed order.
+__global__ void sum_partials_kernel(const float* part, int nz, long long n, float* out, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += part[(size_t)z * n + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
+
+__global__ void silu_kernel(const float* x, float* y, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float v = x[i];
+        y[i] = v * sigm(v);
+    }
+}
+
+// dx = dy * silu'(x)
+__global__ void silu_bwd_kernel(const float* x, const float* dy, float* dx, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float v = x[i], s = sigm(v);
+        dx[i] = dy[i] * (s + v * s * (1.f - s));
+    }
+}
+
+__global__ void add_kernel(float* y, const float* x, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+// out[e] = [h[dst[e]], h[src[e]], ef[e]]  (E x 3W)
+__global__ void edge_concat_kernel(const float* h, const float* ef, const int* src, const int* dst, int E, int W,
+                                   float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)E * 3 * W) return;
+    const int e = (int)(i / (3 * W)), c = (int)(i % (3 * W));
+    float v;
+    if (c < W)
+        v = h[(size_t)dst[e] * W + c];
+    else if (c < 2 * W)
+        v = h[(size_t)src[e] * W + (c - W)];
+    else
+        v = ef[(size_t)e * W + (c - 2 * W)];
+    out[i] = v;
+}
+
+// out[n] = [h[n], a[n]]  (N x 2W)
+__global__ void node_concat_kernel(const float* h, const float* a, int N, int W, float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)N * 2 * W) return;
+    const int n = (int)(i / (2 * W)), c = (int)(i % (2 * W));
+    out[i] = c < W ? h[(size_t)n * W + c] : a[(size_t)n * W + (c - W)];
+}
+
+// agg[n][c] = sum over edges e in [off[n], off[n+1]) of m[perm ? perm[e] : e][c]
+__global__ void segment_sum_kernel(const float* m, const int* off, const int* perm, int N, int W, int ld_in,
+                                   int col0, float* out, int ld_out, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)N * W) return;
+    const int n = (int)(i / W), c = (int)(i % W);
+    float s = 0.f;
+    for (int e = off[n]; e < off[n + 1]; ++e) {
+        const int ee = perm ? perm[e] : e;
+        s += m[(size_t)ee * ld_in + col0 + c];
+    }
+    float* d = out + (size_t)n * ld_out + c;
+    *d = accumulate ? *d + s : s;
+}
+
+// dm[e] = da[dst[e]]
+__global__ void gather_rows_kernel(const float* a, const int* idx, int E, int W, float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)E * W) return;
+    const int e = (int)(i / W), c = (int)(i % W);
+    out[i] = a[(size_t)idx[e] * W + c];
+}
+
+// copy columns [col0, col0 + W) of an (R x ld) array into (R x W), optionally accumulating
+__global__ void take_cols_kernel(const float* in, int R, int ld, int col0, int W, float* out, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)R * W) return;
+    const int r = (int)(i / W), c = (int)(i % W);
+    const float v = in[(size_t)r * ld + col0 + c];
+    out[i] = accumulate ? out[i] + v : v;
+}
+
+// db[j] = sum_i dy[i][j]: block-per-column strided partial sums then a fixed
+// tree; one block per column.
+__global__ __launch_bounds__(256) void colsum_kernel(const float* dy, int R, int C, float* db, int accumulate) {
+    __shared__ float sh[256];
+    const int j = blockIdx.x;
+    float s = 0.f;
+    for (int i = threadIdx.x; i < R; i += 256) s += dy[(size_t)i * C + j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[j] = accumulate ? db[j] + sh[0] : sh[0];
+}
+
+// Adam with bias correction (Kingma & Ba 2015).
+__global__ void adam_kernel(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, float wd) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * p[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Per-graph squared error with the z-reflection ambiguity: for graph g the
+// target is either t or t with z -> -z, whichever is closer.  out[g] =
+// (err+, err-); one block per graph, fixed-order tree.
+__global__ __launch_bounds__(256) void loss_terms_kernel(const float* pred, const float* tgt, const int* goff,
+                                                         float* out) {
+    __shared__ float sp[256], sm[256];
+    const int g = blockIdx.x;
+    float ep = 0.f, em = 0.f;
+    for (int n = goff[g] + threadIdx.x; n < goff[g + 1]; n += 256) {
+        const float dx = pred[3 * n] - tgt[3 * n], dy = pred[3 * n + 1] - tgt[3 * n + 1];
+        const float dzp = pred[3 * n + 2] - tgt[3 * n + 2], dzm = pred[3 * n + 2] + tgt[3 * n + 2];
+        ep += dx * dx + dy * dy + dzp * dzp;
+        em += dx * dx + dy * dy + dzm * dzm;
+    }
+    sp[threadIdx.x] = ep;
+    sm[threadIdx.x] = em;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sp[threadIdx.x] += sp[threadIdx.x + w];
+            sm[threadIdx.x] += sm[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * g] = sp[0];
+        out[2 * g + 1] = sm[0];
+    }
+}
+
+// d loss / d pred for loss = sum_g min(err+, err-) / (3 N_total)
+__global__ void loss_grad_kernel(const float* pred, const float* tgt, const int* gid, const float* terms, int N,
+                                 float scale, float* dpred) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int g = gid[n];
+    const float sgn = terms[2 * g + 1] < terms[2 * g] ? -1.f : 1.f;
+    dpred[3 * n] = 2.f * scale * (pred[3 * n] - tgt[3 * n]);
+    dpred[3 * n + 1] = 2.f * scale * (pred[3 * n + 1] - tgt[3 * n + 1]);
+    dpred[3 * n + 2] = 2.f * scale * (pred[3 * n + 2] - sgn * tgt[3 * n + 2]);
+}
+
+inline unsigned blocks(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void gemm(hipStream_t s, const GemmArgs& a0, float* partials) {
+    GemmArgs a = a0;
+    const int kt = (a.K + BK - 1) / BK;
+    int zs = 1;
+    if (partials && a.ksplit > 1) zs = std::min(a.ksplit, kt);
+    a.kchunk = ((kt + zs - 1) / zs) * BK;
+    zs = (a.K + a.kchunk - 1) / a.kchunk;
+    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, zs);
+    if (zs == 1) {
+        hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, a);
+        return;
+    }
+    GemmArgs p = a;
+    p.C = partials;
+    p.ldc = a.N;
+    p.accumulate = 0;
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, p);
+    // partials are dense M x N; the destination has ldc == N in every split use
+    const long long n = (long long)a.M * a.N;
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(n)), dim3(256), 0, s, (const float*)partials, zs, n, a.C,
+                       a.accumulate);
+}
+
+void silu(hipStream_t s, const float* x, float* y, long long n) {
+    hipLaunchKernelGGL(silu_kernel, dim3(blocks(n)), dim3(256), 0, s, x, y, n);
+}
+void silu_bwd(hipStream_t s, const float* x, const float* dy, float* dx, long long n) {
+    hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks(n)), dim3(256), 0, s, x, dy, dx, n);
+}
+void add(hipStream_t s, float* y, const float* x, long long n) {
+    hipLaunchKernelGGL(add_kernel, dim3(blocks(n)), dim3(256), 0, s, y, x, n);
+}
+void edge_concat(hipStream_t s, const float* h, const float* ef, const int* src, const int* dst, int E, int W,
+                 float* out) {
+    hipLaunchKernelGGL(edge_concat_kernel, dim3(blocks((long long)E * 3 * W)), dim3(256), 0, s, h, ef, src, dst, E, W,
+                       out);
+}
+void node_concat(hipStream_t s, const float* h, const float* a, int N, int W, float* out) {
+    hipLaunchKernelGGL(node_concat_kernel, dim3(blocks((long long)N * 2 * W)), dim3(256), 0, s, h, a, N, W, out);
+}
+void segment_sum(hipStream_t s, const float* m, const int* off, const int* perm, int N, int W, int ld_in, int col0,
+                 float* out, int ld_out, bool accumulate) {
+    hipLaunchKernelGGL(segment_sum_kernel, dim3(blocks((long long)N * W)), dim3(256), 0, s, m, off, perm, N, W, ld_in,
+                       col0, out, ld_out, accumulate ? 1 : 0);
+}
+void gather_rows(hipStream_t s, const float* a, const int* idx, int E, int W, float* out) {
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks((long long)E * W)), dim3(256), 0, s, a, idx, E, W, out);
+}
+void take_cols(hipStream_t s, const float* in, int R, int ld, int col0, int W, float* out, bool accumulate) {
+    hipLaunchKernelGGL(take_cols_kernel, dim3(blocks((long long)R * W)), dim3(256), 0, s, in, R, ld, col0, W, out,
+                       accumulate ? 1 : 0);
+}
+void colsum(hipStream_t s, const float* dy, int R, int C, float* db, bool accumulate) {
+    hipLaunchKernelGGL(colsum_kernel, dim3(C), dim3(256), 0, s, dy, R, C, db, accumulate ? 1 : 0);
+}
+void adam(hipStream_t s, float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+          float eps, int t, float wd) {
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, c1, c2, wd);
+}
+void loss_terms(hipStream_t s, const float* pred, const float* tgt, const int* goff, int G, float* out) {
+    hipLaunchKernelGGL(loss_terms_kernel, dim3(G), dim3(256), 0, s, pred, tgt, goff, out);
+}
+void loss_grad(hipStream_t s, const float* pred, const float* tgt, const int* gid, const float* terms, int N,
+               float scale, float* dpred) {
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks(N)), dim3(256), 0, s, pred, tgt, gid, terms, N, scale, dpred);
+}
+
+}  // namespace nn
+}  // namespace ms
