@@ -340,6 +340,60 @@ int ms_synchronize(ms_ctx* ctx);
  * the average kernel time. */
 int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms);
 
+/* ====================================================== learned warm start
+ * A mesh message-passing network that predicts near-equilibrium node
+ * positions from the flat reference mesh and the programmed metric.  It is
+ * the source of start states that the reference takes from an ansatz file
+ * (move_nodes_to_ansatz_positions.cpp:26-39, read_vtk_data.cpp:380-445); the
+ * damped dynamics always finish the relaxation.  Forward and backward run on
+ * the GPU; GEMMs use bfloat16 matrix-core operands with FP32 accumulation;
+ * parameters, Adam moments and activations are FP32. */
+typedef struct ms_nn ms_nn;
+
+typedef struct ms_nn_arch {
+    int32_t node_features;
+    int32_t edge_features;
+    int32_t width;       /* hidden width of every MLP */
+    int32_t layers;      /* message-passing layers */
+    int32_t out_dim;     /* 3: (u_x, u_y, z) per node */
+    int32_t reserved;
+    uint64_t seed;       /* parameter initialisation */
+} ms_nn_arch;
+
+/* A batch of meshes as one disjoint graph. Directed edges (both directions
+ * of every mesh edge) sorted by destination; node ids are global. */
+typedef struct ms_nn_graph_desc {
+    int32_t num_graphs;
+    int32_t num_nodes;
+    int32_t num_edges;
+    int32_t reserved;
+    const int32_t* node_offsets;  /* num_graphs + 1 */
+    const int32_t* edge_src;      /* num_edges */
+    const int32_t* edge_dst;      /* num_edges, non-decreasing */
+    const float* node_feat;       /* num_nodes x node_features */
+    const float* edge_feat;       /* num_edges x edge_features */
+} ms_nn_graph_desc;
+
+int ms_nn_create(const ms_nn_arch* arch, ms_nn** out);
+void ms_nn_destroy(ms_nn* nn);
+const char* ms_nn_last_error(const ms_nn* nn);
+int64_t ms_nn_num_params(const ms_nn* nn);
+int ms_nn_get_params(ms_nn* nn, float* params);
+/* Also resets the Adam moments. */
+int ms_nn_set_params(ms_nn* nn, const float* params);
+int ms_nn_set_graph(ms_nn* nn, const ms_nn_graph_desc* g);
+/* Predictions for the current graph (num_nodes x 3); out may be NULL. */
+int ms_nn_forward(ms_nn* nn, float* out);
+/* After ms_nn_forward: loss = sum over graphs of the smaller of the squared
+ * errors against target and against target reflected in z, / (3 num_nodes),
+ * and its gradient w.r.t. every parameter (grads may be NULL). */
+int ms_nn_loss_backward(ms_nn* nn, const float* target, float* loss, float* grads);
+int ms_nn_adam_step(ms_nn* nn, float lr, float beta1, float beta2, float eps, float weight_decay);
+/* Measurement/test aid: C (M x N) = A @ B through the network's GEMM kernel
+ * (A row-major M x K, or K x M read transposed; split-K over ksplit blocks). */
+int ms_nn_gemm_test(int32_t M, int32_t N, int32_t K, int32_t transpose_a, int32_t ksplit, const float* A,
+                    const float* B, float* C);
+
 #ifdef __cplusplus
 }
 #endif

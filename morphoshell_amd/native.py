@@ -28,7 +28,9 @@ EXPORTED = [
     "ms_calc_geometry", "ms_calc_energy_densities", "ms_get_forces", "ms_set_forces", "ms_step",
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
     "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_relax_setup",
-    "ms_relax_run",
+    "ms_relax_run", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
+    "ms_nn_set_params", "ms_nn_set_graph", "ms_nn_forward", "ms_nn_loss_backward", "ms_nn_adam_step",
+    "ms_nn_gemm_test",
 ]
 
 
@@ -66,8 +68,23 @@ class ShellParams(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 _u8p = C.POINTER(C.c_uint8)
+
+
+class NNArch(C.Structure):
+    _fields_ = [
+        ("node_features", C.c_int32), ("edge_features", C.c_int32), ("width", C.c_int32), ("layers", C.c_int32),
+        ("out_dim", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64),
+    ]
+
+
+class NNGraphDesc(C.Structure):
+    _fields_ = [
+        ("num_graphs", C.c_int32), ("num_nodes", C.c_int32), ("num_edges", C.c_int32), ("reserved", C.c_int32),
+        ("node_offsets", _ip), ("edge_src", _ip), ("edge_dst", _ip), ("node_feat", _fp), ("edge_feat", _fp),
+    ]
 
 
 class ShellDesc(C.Structure):
@@ -173,6 +190,17 @@ def lib() -> C.CDLL:
         "ms_calibration_copy": (C.c_int, [C.c_int64, C.c_int32, _dp]),
         "ms_relax_setup": (C.c_int, [P, C.c_int32, C.POINTER(RelaxParams)]),
         "ms_relax_run": (C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(RelaxReport)]),
+        "ms_nn_create": (C.c_int, [C.POINTER(NNArch), C.POINTER(P)]),
+        "ms_nn_destroy": (None, [P]),
+        "ms_nn_last_error": (C.c_char_p, [P]),
+        "ms_nn_num_params": (C.c_int64, [P]),
+        "ms_nn_get_params": (C.c_int, [P, _fp]),
+        "ms_nn_set_params": (C.c_int, [P, _fp]),
+        "ms_nn_set_graph": (C.c_int, [P, C.POINTER(NNGraphDesc)]),
+        "ms_nn_forward": (C.c_int, [P, _fp]),
+        "ms_nn_loss_backward": (C.c_int, [P, _fp, _fp, _fp]),
+        "ms_nn_adam_step": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "ms_nn_gemm_test": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, _fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -200,6 +228,13 @@ def dptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)
 
 
 def iptr(a: np.ndarray | None):

@@ -1,0 +1,391 @@
+"""Learned warm start: a mesh message-passing network that predicts
+near-equilibrium 3-D node positions from the flat reference mesh and the
+programmed metric (abar, bbar, h).
+
+It replaces the ansatz start of the reference
+(move_nodes_to_ansatz_positions.cpp:26-39 with positions read by
+read_vtk_data.cpp:380-445): the predicted positions become the start state
+at dial factor 1, the plain damped dynamics finish the relaxation and judge
+it by residual force, and a flat start with noise is used instead whenever
+the prediction has a higher elastic energy than that flat start.
+
+The network itself (forward, backward, Adam) runs on the GPU in the native
+library (``ms_nn_*``, BF16 matrix-core GEMMs with FP32 accumulation).  This
+module builds graphs and features, generates training equilibria with the
+batched solver, trains, and applies the model.
+
+Features per node (10): centred reference position / L (2), boundary flag,
+mean over incident triangles of (abar_xx - 1, abar_xy, abar_yy - 1) and of
+bbar * L / 4 (3), and h / l; per directed edge (3): (dx, dy, |d|) / l, with
+L the largest distance from the centroid and l the mean edge length.
+Output per node: 10 * (displacement / L) in the frame that aligns the
+equilibrium with the reference (centroid and Kabsch rotation removed); the
+up/down reflection is left to the loss (min over z -> -z).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import native as nat
+
+NODE_FEATURES = 10
+EDGE_FEATURES = 3
+WIDTH = 64
+LAYERS = 4
+OUT_SCALE = 10.0
+
+
+def log(msg: str) -> None:
+    print(f"[warmstart] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ graphs
+
+def final_metric(sh) -> tuple[np.ndarray, np.ndarray]:
+    """abar, bbar (T x 3) the shell is programmed to reach at dial factor 1."""
+    m = sh.mesh
+    if sh.settings.dial_protocol == 3:
+        ang, lam, nu = m.abar_info[:, 0], m.abar_info[:, 1], m.abar_info[:, 2]
+        c, s = np.cos(ang), np.sin(ang)
+        l2, lp = lam * lam, lam ** (-2.0 * nu)
+        ab = np.stack([l2 * c * c + lp * s * s, (l2 - lp) * c * s, lp * c * c + l2 * s * s], axis=1)
+        return ab, np.zeros_like(ab)
+    return np.asarray(m.abar_info[:, :3], float), np.asarray(m.bbar_info[:, :3], float)
+
+
+def shell_graph(sh) -> dict:
+    m = sh.mesh
+    N, T = sh.N, sh.T
+    tris = m.tris.astype(np.int64)
+    pairs = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]], axis=0)
+    pairs = np.concatenate([pairs, pairs[:, ::-1]], axis=0)
+    key = np.unique(pairs[:, 1] * N + pairs[:, 0])  # sorted by (dst, src)
+    dst = (key // N).astype(np.int32)
+    src = (key % N).astype(np.int32)
+    xy = m.points[:, :2]
+    cen = xy.mean(axis=0)
+    rel = xy - cen
+    L = float(np.sqrt((rel ** 2).sum(1)).max())
+    d = xy[dst] - xy[src]
+    ell = float(np.sqrt((d ** 2).sum(1)).mean())
+    ab, bb = final_metric(sh)
+    # per-node means over incident triangles
+    cnt = np.bincount(tris.ravel(), minlength=N).astype(float)
+    def node_mean(v):
+        acc = np.zeros(N)
+        for k in range(3):
+            np.add.at(acc, tris[:, k], v)
+        return acc / cnt
+    h = sh.thickness
+    feat = np.zeros((N, NODE_FEATURES), np.float32)
+    feat[:, 0:2] = rel / L
+    feat[:, 2] = sh.info["boundary_nodes"].astype(np.float32)
+    feat[:, 3] = node_mean(ab[:, 0] - 1.0)
+    feat[:, 4] = node_mean(ab[:, 1])
+    feat[:, 5] = node_mean(ab[:, 2] - 1.0)
+    for k in range(3):
+        feat[:, 6 + k] = node_mean(bb[:, k] * L / 4.0)
+    feat[:, 9] = node_mean(h / ell)
+    ef = np.zeros((len(src), EDGE_FEATURES), np.float32)
+    ef[:, 0:2] = d / ell
+    ef[:, 2] = np.sqrt((d ** 2).sum(1)) / ell
+    return {"N": N, "src": src, "dst": dst, "node_feat": feat, "edge_feat": ef, "L": L, "ell": ell,
+            "center": cen}
+
+
+def batch_graphs(graphs: list[dict]) -> dict:
+    offs = np.zeros(len(graphs) + 1, np.int32)
+    srcs, dsts, nfs, efs = [], [], [], []
+    for i, g in enumerate(graphs):
+        offs[i + 1] = offs[i] + g["N"]
+        srcs.append(g["src"] + offs[i])
+        dsts.append(g["dst"] + offs[i])
+        nfs.append(g["node_feat"])
+        efs.append(g["edge_feat"])
+    return {"offsets": offs, "src": np.concatenate(srcs).astype(np.int32), "dst": np.concatenate(dsts).astype(np.int32),
+            "node_feat": np.ascontiguousarray(np.concatenate(nfs), np.float32),
+            "edge_feat": np.ascontiguousarray(np.concatenate(efs), np.float32)}
+
+
+def kabsch_target(sh, graph: dict, dofs: np.ndarray) -> np.ndarray:
+    """Network target of an equilibrium: 10 * displacement / L after removing
+    the rigid motion that best maps the equilibrium onto the flat reference."""
+    N = sh.N
+    P = dofs.reshape(3, N).T.copy()
+    R = sh.mesh.points.copy()
+    Pc, Rc = P.mean(0), R.mean(0)
+    H = (P - Pc).T @ (R - Rc)
+    U, _, Vt = np.linalg.svd(H)
+    D = np.diag([1.0, 1.0, np.sign(np.linalg.det(Vt.T @ U.T))])
+    Rot = Vt.T @ D @ U.T
+    Pa = (P - Pc) @ Rot.T + Rc
+    disp = Pa - R
+    return (OUT_SCALE * disp / graph["L"]).astype(np.float32)
+
+
+def positions_from_output(sh, graph: dict, out: np.ndarray) -> np.ndarray:
+    """dofs (3N, SoA) of the reference positions moved by the network output."""
+    disp = out.astype(np.float64) * graph["L"] / OUT_SCALE
+    P = sh.mesh.points + disp
+    return np.ascontiguousarray(P.T.reshape(-1))
+
+
+# --------------------------------------------------------------- the model
+
+class WarmStartNet:
+    def __init__(self, seed: int = 0, width: int = WIDTH, layers: int = LAYERS):
+        a = nat.NNArch()
+        a.node_features, a.edge_features, a.width, a.layers, a.out_dim, a.seed = (
+            NODE_FEATURES, EDGE_FEATURES, width, layers, 3, seed)
+        self.arch = (width, layers)
+        self.L = nat.lib()
+        h = C.c_void_p()
+        nat.check(self.L.ms_nn_create(C.byref(a), C.byref(h)))
+        self.h = h
+        self.n = int(self.L.ms_nn_num_params(h))
+        self._graph = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ms_nn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, code):
+        if code != nat.MS_OK:
+            raise nat.NativeError(code, (self.L.ms_nn_last_error(self.h) or b"").decode())
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.n, np.float32)
+        self._ck(self.L.ms_nn_get_params(self.h, nat.fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n
+        self._ck(self.L.ms_nn_set_params(self.h, nat.fptr(p)))
+
+    def save(self, path: str) -> None:
+        np.savez_compressed(path, params=self.params(), width=self.arch[0], layers=self.arch[1])
+
+    @classmethod
+    def load(cls, path: str) -> "WarmStartNet":
+        z = np.load(path)
+        net = cls(width=int(z["width"]), layers=int(z["layers"]))
+        net.set_params(z["params"])
+        return net
+
+    def set_graph(self, b: dict) -> None:
+        self._graph = b  # keep the host arrays alive for the call
+        d = nat.NNGraphDesc()
+        d.num_graphs = len(b["offsets"]) - 1
+        d.num_nodes = int(b["offsets"][-1])
+        d.num_edges = len(b["src"])
+        d.node_offsets = nat.iptr(b["offsets"])
+        d.edge_src = nat.iptr(b["src"])
+        d.edge_dst = nat.iptr(b["dst"])
+        d.node_feat = nat.fptr(b["node_feat"])
+        d.edge_feat = nat.fptr(b["edge_feat"])
+        self._ck(self.L.ms_nn_set_graph(self.h, C.byref(d)))
+
+    def forward(self) -> np.ndarray:
+        n = int(self._graph["offsets"][-1])
+        out = np.empty((n, 3), np.float32)
+        self._ck(self.L.ms_nn_forward(self.h, nat.fptr(out)))
+        return out
+
+    def loss_backward(self, target: np.ndarray, want_grads: bool = False):
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        g = np.empty(self.n, np.float32) if want_grads else None
+        self._ck(self.L.ms_nn_loss_backward(self.h, nat.fptr(t), C.byref(loss), nat.fptr(g)))
+        return (loss.value, g) if want_grads else loss.value
+
+    def train_step(self, target: np.ndarray, lr: float, wd: float = 0.0) -> float:
+        self._ck(self.L.ms_nn_forward(self.h, None))
+        loss = self.loss_backward(target)
+        self._ck(self.L.ms_nn_adam_step(self.h, lr, 0.9, 0.999, 1e-8, wd))
+        return loss
+
+    def predict_dofs(self, sh, graph: dict | None = None) -> np.ndarray:
+        g = graph or shell_graph(sh)
+        self.set_graph(batch_graphs([g]))
+        return positions_from_output(sh, g, self.forward())
+
+
+# ------------------------------------------------------------ data + train
+
+def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000):
+    """Small thick shells that relax in ~1e5 steps: discs/squares, GRF lambda
+    (amplitude 0.05-0.15, clipped at 3 sigma), bbar = kappa I (|kappa| <= 1.5),
+    h in [0.02, 0.05]."""
+    from .meshing import disc_mesh, grf, isotropic_abar_info, square_mesh, uniform_bbar_info
+    from .shell import Settings, prepare_shell
+
+    out = []
+    for i in range(num):
+        rng = np.random.default_rng([seed, i, 7])
+        T = math.exp(rng.uniform(math.log(tmin), math.log(tmax)))
+        if rng.uniform() < 0.5:
+            m = disc_mesh(math.sqrt(7.26 / T), seed=seed * 1009 + i)
+        else:
+            m = square_mesh(math.sqrt(9.24 / T), seed=seed * 1009 + i)
+        c = m.centroids()
+        amp = rng.uniform(0.05, 0.15)
+        m.abar_info = isotropic_abar_info(1.0 + amp * np.clip(grf(c[:, :2], 0.3, 64, seed * 7907 + i), -3, 3))
+        m.bbar_info = uniform_bbar_info(m.num_tris, rng.uniform(-1.5, 1.5))
+        m.fill_defaults(h=rng.uniform(0.02, 0.05))
+        out.append(prepare_shell(m, Settings(def_poisson_ratio=0.5)))
+    return out
+
+
+def relax_batch(shells, dial_steps: int, tol: float, max_steps: int, starts=None, dial_start: float = 0.0,
+                precision: str = "mixed", poll: int = 5000, equil_interval: int = 100):
+    """Relax a batch with the device-run protocol; returns (reports, end dofs, seconds)."""
+    from .shell import Context
+    from .workloads import flat_noisy_state
+
+    t0 = time.perf_counter()
+    with Context(shells, precision=precision) as ctx:
+        for i, sh in enumerate(shells):
+            x, v = flat_noisy_state(sh, seed=i) if starts is None or starts[i] is None else (starts[i], None)
+            ctx.set_state(i, x, v)
+            ctx.relax_setup(i, steps_per_dial_phase=dial_steps, dial_spacing=1.0, dial_start=dial_start,
+                            equil_interval=equil_interval, tol_f=tol, tol_v=tol)
+        reps = ctx.relax_run(max_steps, poll_every=poll)
+        ends = [ctx.get_state(i)[0] for i in range(len(shells))]
+    return reps, ends, time.perf_counter() - t0
+
+
+def generate(num: int, seed: int, batch: int = 128, dial_steps: int = 2000, tol: float = 1e-5,
+             max_steps: int = 600000) -> list[tuple]:
+    """Equilibria of the training family produced by the batched solver."""
+    data = []
+    for b0 in range(0, num, batch):
+        shells = training_family(min(batch, num - b0), seed + b0)
+        reps, ends, secs = relax_batch(shells, dial_steps, tol, max_steps)
+        ok = [i for i, r in enumerate(reps) if r["status"] == 2]
+        log(f"batch {b0 // batch}: {len(ok)}/{len(shells)} equilibria in {secs:.1f}s")
+        data += [(shells[i], ends[i]) for i in ok]
+    return data
+
+
+def train(net: WarmStartNet, data: list[tuple], epochs: int = 60, graphs_per_step: int = 8, lr: float = 2e-3,
+          seed: int = 0) -> list[float]:
+    graphs = [shell_graph(sh) for sh, _ in data]
+    targets = [kabsch_target(sh, g, x) for (sh, x), g in zip(data, graphs)]
+    rng = np.random.default_rng(seed)
+    hist = []
+    steps = epochs * max(1, len(data) // graphs_per_step)
+    k = 0
+    for ep in range(epochs):
+        order = rng.permutation(len(data))
+        losses = []
+        for j in range(0, len(order) - graphs_per_step + 1, graphs_per_step):
+            idx = order[j:j + graphs_per_step]
+            net.set_graph(batch_graphs([graphs[i] for i in idx]))
+            lr_k = lr * 0.5 * (1.0 + math.cos(math.pi * k / max(1, steps)))  # cosine decay
+            losses.append(net.train_step(np.concatenate([targets[i] for i in idx]), lr_k))
+            k += 1
+        hist.append(float(np.mean(losses)) if losses else float("nan"))
+        if ep % 10 == 0 or ep == epochs - 1:
+            log(f"epoch {ep}: loss {hist[-1]:.5f}")
+    return hist
+
+
+def warm_start_states(net: WarmStartNet, shells, energies_of) -> tuple[list, list]:
+    """Predicted start states with the energy fallback.  energies_of(i, dofs)
+    returns the elastic energy of shell i at dial factor 1."""
+    from .workloads import flat_noisy_state
+
+    starts, used = [], []
+    for i, sh in enumerate(shells):
+        pred = net.predict_dofs(sh)
+        flat, _ = flat_noisy_state(sh, seed=i)
+        if energies_of(i, pred) < energies_of(i, flat):
+            starts.append(pred)
+            used.append(True)
+        else:
+            starts.append(flat)
+            used.append(False)
+    return starts, used
+
+
+def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: int = 2000, tol: float = 1e-5,
+             max_steps: int = 600000) -> dict:
+    """Steps to equilibrium from the learned warm start (at dial factor 1)
+    against the flat start with noise dialled as usual, on unseen seeds."""
+    from .shell import Context
+
+    shells = training_family(num, seed)
+    with Context(shells, precision="fp64") as ctx:
+        def energy(i, dofs):
+            ctx.set_state(i, dofs)
+            e = ctx.calc_energies(i)
+            return e[0] + e[1]
+        starts, used = warm_start_states(net, shells, energy)
+    base, base_end, t_base = relax_batch(shells, dial_steps, tol, max_steps)
+    warm, warm_end, t_warm = relax_batch(shells, 1, tol, max_steps, starts=starts, dial_start=1.0)
+    with Context(shells, precision="fp64") as ctx:
+        e_base, e_warm = [], []
+        for i in range(num):
+            ctx.set_state(i, base_end[i])
+            e = ctx.calc_energies(i)
+            e_base.append(e[0] + e[1])
+            ctx.set_state(i, warm_end[i])
+            e = ctx.calc_energies(i)
+            e_warm.append(e[0] + e[1])
+    both = [i for i in range(num) if base[i]["status"] == 2 and warm[i]["status"] == 2]
+    sb = np.array([base[i]["eq_step"] + 1 for i in both], float)
+    sw = np.array([warm[i]["eq_step"] + 1 for i in both], float)
+    de = np.array([abs(e_warm[i] - e_base[i]) / max(abs(e_base[i]), 1e-300) for i in both])
+    return {
+        "shells": num, "both_converged": len(both), "prediction_used": int(sum(used)),
+        "steps_flat_mean": float(sb.mean()) if len(sb) else None,
+        "steps_warm_mean": float(sw.mean()) if len(sw) else None,
+        "steps_ratio_median": float(np.median(sb / sw)) if len(sb) else None,
+        "seconds_flat": t_base, "seconds_warm": t_warm,
+        "final_energy_rel_diff_median": float(np.median(de)) if len(de) else None,
+        "final_energy_rel_diff_max": float(de.max()) if len(de) else None,
+        "tolerance": tol,
+    }
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="generate equilibria, train the warm-start network, evaluate")
+    ap.add_argument("--num-train", type=int, default=512)
+    ap.add_argument("--num-eval", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--out", default="warmstart_params.npz")
+    ap.add_argument("--max-steps", type=int, default=600000)
+    args = ap.parse_args(argv)
+    t0 = time.time()
+    data = generate(args.num_train, seed=1, max_steps=args.max_steps)
+    t_gen = time.time() - t0
+    net = WarmStartNet(seed=0)
+    t0 = time.time()
+    hist = train(net, data, epochs=args.epochs)
+    t_train = time.time() - t0
+    net.save(args.out)
+    ev = evaluate(net, args.num_eval, max_steps=args.max_steps)
+    print(json.dumps({"equilibria": len(data), "generate_s": t_gen, "train_s": t_train,
+                      "loss_first": hist[0], "loss_last": hist[-1], "eval": ev}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,98 @@
+"""GPU checks of the warm-start network against its CPU reference.
+
+* GEMM (bfloat16 matrix-core operands, FP32 accumulation) against numpy with
+  the same operand rounding and FP64 accumulation: <= 2e-5 relative.
+* Forward and gradients of the whole network against oracle/mpnn_ref.py with
+  bfloat16 operand rounding: outputs <= 5e-3 relative, gradients cosine >=
+  0.999 and norm error <= 2e-2 (activations are FP32 on the GPU and FP64 in
+  the reference, so a few bfloat16 roundings land on the other side).
+* Training is deterministic and fits a fixed batch.
+"""
+import numpy as np
+import pytest
+
+from oracle import mpnn_ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("M,N,K,ta,ks", [(64, 64, 16, 0, 1), (100, 70, 37, 0, 1), (33, 64, 192, 0, 1),
+                                         (192, 64, 5000, 1, 8), (10, 3, 64, 0, 1)])
+def test_gemm_matches_bf16_reference(M, N, K, ta, ks):
+    import ctypes as C
+
+    from morphoshell_amd import native as nat
+
+    rng = np.random.default_rng(M * 7 + K)
+    A = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    Cm = np.zeros((M, N), np.float32)
+    nat.check(nat.lib().ms_nn_gemm_test(M, N, K, ta, ks, nat.fptr(np.ascontiguousarray(A)), nat.fptr(B),
+                                        nat.fptr(Cm)))
+    Aop = mpnn_ref.bf16(A.T if ta else A)
+    ref = Aop @ mpnn_ref.bf16(B)
+    err = np.abs(Cm - ref).max() / np.abs(ref).max()
+    assert err <= 2e-5, err
+    del C
+
+
+def _small_batch():
+    from morphoshell_amd.meshing import disc_mesh, grf, isotropic_abar_info, uniform_bbar_info
+    from morphoshell_amd.shell import Settings, prepare_shell
+    from morphoshell_amd.warmstart import batch_graphs, shell_graph
+
+    shells = []
+    for i in range(2):
+        m = disc_mesh(0.15, seed=i)
+        m.abar_info = isotropic_abar_info(1.0 + 0.1 * grf(m.centroids()[:, :2], 0.3, seed=i))
+        m.bbar_info = uniform_bbar_info(m.num_tris, 0.5 - i)
+        m.fill_defaults(h=0.03)
+        shells.append(prepare_shell(m, Settings()))
+    graphs = [shell_graph(s) for s in shells]
+    return shells, graphs, batch_graphs(graphs)
+
+
+def test_forward_and_gradients_match_reference():
+    from morphoshell_amd.warmstart import EDGE_FEATURES, NODE_FEATURES, WarmStartNet
+
+    shells, graphs, b = _small_batch()
+    net = WarmStartNet(seed=3, width=32, layers=2)
+    p = net.params().astype(np.float64)
+    net.set_graph(b)
+    out = net.forward()
+    rng = np.random.default_rng(0)
+    tgt = rng.normal(size=out.shape).astype(np.float32) * 0.3
+    loss_g, grad_g = net.loss_backward(tgt, want_grads=True)
+    ref = mpnn_ref.Net(NODE_FEATURES, EDGE_FEATURES, 32, 2, bf16_ops=True)
+    c = {}
+    out_r = ref.forward(p, b["node_feat"].astype(np.float64), b["edge_feat"].astype(np.float64), b["src"], b["dst"],
+                        c)
+    loss_r, dout = ref.loss(out_r, tgt.astype(np.float64), b["offsets"])
+    grad_r = ref.backward(p, b["node_feat"].astype(np.float64), b["edge_feat"].astype(np.float64), b["src"],
+                          b["dst"], c, dout)
+    assert np.abs(out - out_r).max() <= 5e-3 * np.abs(out_r).max()
+    assert loss_g == pytest.approx(loss_r, rel=5e-3)
+    cos = float(grad_g @ grad_r / (np.linalg.norm(grad_g) * np.linalg.norm(grad_r)))
+    nerr = float(np.linalg.norm(grad_g - grad_r) / np.linalg.norm(grad_r))
+    print(f"gradient cosine {cos:.6f}, norm error {nerr:.2e}")
+    assert cos >= 0.999 and nerr <= 2e-2
+
+
+def test_training_is_deterministic_and_fits():
+    from morphoshell_amd.warmstart import WarmStartNet
+
+    shells, graphs, b = _small_batch()
+    rng = np.random.default_rng(1)
+    N = int(b["offsets"][-1])
+    xy = b["node_feat"][:, :2]
+    tgt = np.stack([0.3 * xy[:, 0] * xy[:, 1], -0.2 * xy[:, 0], 0.5 * (xy ** 2).sum(1)], 1).astype(np.float32)
+    params = []
+    for run in range(2):
+        net = WarmStartNet(seed=5, width=32, layers=2)
+        net.set_graph(b)
+        losses = [net.train_step(tgt, 3e-3) for _ in range(150)]
+        params.append(net.params())
+        net.close()
+    assert np.array_equal(params[0], params[1])
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+    del rng, N
