@@ -318,7 +318,12 @@ typedef struct ms_tuning {
     int32_t tri_grid_per_cu;  /* persistent grid: blocks per CU; 0 = one block per chunk */
     int32_t node_grid_per_cu;
     int32_t graph_steps;      /* steps captured per HIP graph */
-    int32_t reserved[3];
+    int32_t fused;            /* 1 = one owner-computes tile launch per step (a
+                                 tile is a node chunk; its workgroup recomputes
+                                 the halo triangles and gathers from LDS);
+                                 0 = triangle launch + node launch; -1 keeps.
+                                 Both give bit-identical states. */
+    int32_t reserved[2];
 } ms_tuning;
 int ms_set_tuning(ms_ctx* ctx, const ms_tuning* t);
 int ms_get_tuning(const ms_ctx* ctx, ms_tuning* t);

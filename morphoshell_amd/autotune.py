@@ -1,7 +1,8 @@
 """Launch-shape autotuner of the per-step kernels.
 
-Searches triangles-per-block, nodes-per-block, persistent-grid occupancy and
-steps per captured HIP graph for a context, timing each candidate with the
+Searches triangles-per-block, nodes-per-block, persistent-grid occupancy,
+steps per captured HIP graph and the step form (triangle + node launches, or
+the fused owner-computes tile launch) for a context, timing each candidate with the
 library's own HIP events, and caches the winner by workload signature (total
 triangles and nodes bucketed to powers of two, shell count bucket, precision,
 device) in a JSON file (``$MS_TUNE_CACHE`` or
@@ -25,7 +26,20 @@ DEFAULT_SPACE = {
     "node_block": (128, 256),
     "grid_per_cu": (0, 4, 8),
     "graph_steps": (16,),
+    "fused_node_block": (64, 128, 256),  # tile sizes of the fused step
 }
+
+
+def _candidates(sp: dict):
+    for tb, nb, g, gs in itertools.product(sp["tri_block"], sp["node_block"], sp["grid_per_cu"], sp["graph_steps"]):
+        yield {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs, "fused": 0}
+    for nb, g, gs in itertools.product(sp.get("fused_node_block", ()), sp["grid_per_cu"], sp["graph_steps"]):
+        yield {"tri_block": 256, "node_block": nb, "grid_per_cu": g, "graph_steps": gs, "fused": 1}
+
+
+def _apply(ctx: Context, c: dict) -> None:
+    ctx.set_tuning(tri_block=c["tri_block"], node_block=c["node_block"], tri_grid_per_cu=c["grid_per_cu"],
+                   node_grid_per_cu=c["grid_per_cu"], graph_steps=c["graph_steps"], fused=c.get("fused", 0))
 
 
 def _cache_path() -> str:
@@ -41,7 +55,7 @@ def signature(ctx: Context) -> str:
     S = len(ctx.shells)
     b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
     prec = "mixed" if ctx.precision == 1 else "fp64"
-    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-v2"
 
 
 def _load() -> dict:
@@ -85,22 +99,19 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
     cache = _load() if use_cache else {}
     if sig in cache:
         best = cache[sig]
-        ctx.set_tuning(tri_block=best["tri_block"], node_block=best["node_block"],
-                       tri_grid_per_cu=best["grid_per_cu"], node_grid_per_cu=best["grid_per_cu"],
-                       graph_steps=best["graph_steps"])
+        _apply(ctx, best)
         return dict(best, cached=True)
     sp = space or DEFAULT_SPACE
     results = []
-    for tb, nb, g, gs in itertools.product(sp["tri_block"], sp["node_block"], sp["grid_per_cu"], sp["graph_steps"]):
-        ctx.set_tuning(tri_block=tb, node_block=nb, tri_grid_per_cu=g, node_grid_per_cu=g, graph_steps=gs)
+    for cand in _candidates(sp):
+        _apply(ctx, cand)
         t = time_config(ctx, steps)
-        results.append((t, {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs}))
+        results.append((t, cand))
         if verbose:
-            print(f"[autotune] {sig} tri {tb} node {nb} grid/cu {g} graph {gs}: {t * 1e3:.4f} ms/step", file=sys.stderr)
+            print(f"[autotune] {sig} {cand}: {t * 1e3:.4f} ms/step", file=sys.stderr)
     t, best = min(results, key=lambda r: r[0])
     best = dict(best, ms_per_step=t * 1e3, signature=sig, candidates=len(results))
-    ctx.set_tuning(tri_block=best["tri_block"], node_block=best["node_block"], tri_grid_per_cu=best["grid_per_cu"],
-                   node_grid_per_cu=best["grid_per_cu"], graph_steps=best["graph_steps"])
+    _apply(ctx, best)
     if use_cache:
         cache[sig] = best
         _store(cache)

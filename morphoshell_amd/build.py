@@ -35,6 +35,7 @@ SOURCES = [
     ("context.cpp", "hip", []),
     ("tri_kernels.hip", "hip", []),
     ("node_kernels.hip", "hip", ["-ffp-contract=off"]),
+    ("tile_kernels.hip", "hip", []),
     ("nn_kernels.hip", "hip", []),
     ("nn_model.cpp", "hip", []),
 ]

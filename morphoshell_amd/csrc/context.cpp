@@ -53,6 +53,13 @@ struct ms_ctx {
     int n_tri_chunks = 0, n_node_chunks = 0;
     size_t part_len = 0;
 
+    // fused tile step (built on demand for the current node chunks)
+    int* tile_off = nullptr;
+    int* tile_tris = nullptr;
+    unsigned short* tile_dst = nullptr;
+    int tile_max_entries = 0;
+    bool tiles_ready = false;
+
     // graphs
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
@@ -123,6 +130,82 @@ int build_chunks(ms_ctx* c) {
     c->part_len = 4 * (size_t)std::max(tc.size(), nc.size());
     HIPC(c, hipMalloc((void**)&c->b.part, c->part_len * sizeof(double)));
     HIPC(c, hipMalloc((void**)&c->b.part_i, c->part_len * sizeof(int)));
+    c->tiles_ready = false;
+    return MS_OK;
+}
+
+void free_tiles(ms_ctx* c) {
+    if (c->tile_off) hipFree(c->tile_off);
+    if (c->tile_tris) hipFree(c->tile_tris);
+    if (c->tile_dst) hipFree(c->tile_dst);
+    c->tile_off = c->tile_tris = nullptr;
+    c->tile_dst = nullptr;
+    c->tile_max_entries = 0;
+    c->tiles_ready = false;
+}
+
+// Tile metadata of the fused step for the current node chunks: per tile the
+// ascending list of triangles whose patch touches one of its nodes, and for
+// each (triangle, slot) the LDS position of its patch-incidence entry.
+int build_tiles(ms_ctx* c) {
+    free_tiles(c);
+    const int T = c->T, N = c->N;
+    std::vector<int> pid((size_t)6 * T), ent;
+    std::vector<long long> off((size_t)N + 1);
+    std::vector<Chunk> nc(c->n_node_chunks);
+    HIPC(c, hipMemcpy(pid.data(), c->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPC(c, hipMemcpy(off.data(), c->csr_off, off.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    ent.resize((size_t)off[N]);
+    HIPC(c, hipMemcpy(ent.data(), c->csr_ent, ent.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPC(c, hipMemcpy(nc.data(), c->d_node_chunks, nc.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
+    std::vector<int> toff(nc.size() + 1, 0), tris;
+    std::vector<unsigned short> dst;
+    int max_entries = 0;
+    for (size_t ci = 0; ci < nc.size(); ++ci) {
+        const int n0 = nc[ci].x, n1 = nc[ci].y;
+        const long long base = off[n0];
+        const long long cnt = off[n1] - base;
+        if (cnt >= 0xFFFF)
+            return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu has %lld patch entries (limit 65534)", ci, cnt));
+        max_entries = std::max(max_entries, (int)cnt);
+        std::vector<int> tl;
+        tl.reserve((size_t)cnt);
+        for (long long e = base; e < off[n1]; ++e) tl.push_back(ent[e] / 6);
+        std::sort(tl.begin(), tl.end());
+        tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+        for (int t : tl) {
+            tris.push_back(t);
+            for (int s = 0; s < 6; ++s) {
+                const int p = pid[(size_t)s * T + t];
+                unsigned short q = 0xFFFF;
+                if (p >= n0 && p < n1) {
+                    const int key = t * 6 + s;
+                    const int* b = ent.data() + off[p];
+                    const int* e = ent.data() + off[p + 1];
+                    const int* it = std::lower_bound(b, e, key);
+                    if (it == e || *it != key)
+                        return fail(c, MS_ERR_ARG, "fused step: patch-incidence table is inconsistent");
+                    q = (unsigned short)((it - ent.data()) - base);
+                }
+                dst.push_back(q);
+            }
+        }
+        toff[ci + 1] = (int)tris.size();
+    }
+    const int lds64 = tile_lds_bytes(max_entries, false), lds32 = tile_lds_bytes(max_entries, true);
+    if (lds64 > 160 * 1024)
+        return fail(c, MS_ERR_ARG, fmt("fused step: a tile needs %d bytes of LDS (limit 160 KiB)", lds64));
+    HIPC(c, prepare_tile_kernels(lds64, lds32));
+    HIPC(c, hipMalloc((void**)&c->tile_off, toff.size() * sizeof(int)));
+    HIPC(c, hipMalloc((void**)&c->tile_tris, std::max<size_t>(1, tris.size()) * sizeof(int)));
+    HIPC(c, hipMalloc((void**)&c->tile_dst, std::max<size_t>(1, dst.size()) * sizeof(unsigned short)));
+    HIPC(c, hipMemcpy(c->tile_off, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!tris.empty()) {
+        HIPC(c, hipMemcpy(c->tile_tris, tris.data(), tris.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPC(c, hipMemcpy(c->tile_dst, dst.data(), dst.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+    }
+    c->tile_max_entries = max_entries;
+    c->tiles_ready = true;
     return MS_OK;
 }
 
@@ -195,14 +278,31 @@ void params_to_dev(const ms_shell_params& p, ShellDev& S) {
     S.from_ansatz = p.dial_from_ansatz ? 1 : 0;
 }
 
-void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v) {
+TileData tile_data(const ms_ctx* c) {
+    return TileData{c->tile_off, c->tile_tris, c->tile_dst, c->tile_max_entries};
+}
+
+// One step; with the fused tuning the tile launch replaces the triangle and
+// node launches.  timed: events around the two parts (ev[0..2]).
+void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v, bool timed = false) {
     const bool mixed = c->precision == MS_MIXED;
     TriData td = tri_data(c);
     NodeData nd = node_data(c);
-    launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, true);
     StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
-    launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
-                     c->d_clock, mixed, fl);
+    if (timed) hipEventRecord(c->ev[0], c->stream);
+    if (c->tune.fused == 1) {
+        launch_tile_step(node_launch(c), td, nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks,
+                         c->d_shells, mixed, fl);
+        if (timed) hipEventRecord(c->ev[1], c->stream);
+        launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
+                                c->d_clock, fl);
+    } else {
+        launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, true);
+        if (timed) hipEventRecord(c->ev[1], c->stream);
+        launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
+                         c->d_clock, mixed, fl);
+    }
+    if (timed) hipEventRecord(c->ev[2], c->stream);
 }
 
 void destroy_graph(ms_ctx* c) {
@@ -282,6 +382,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     c->tune.tri_grid_per_cu = 0;
     c->tune.node_grid_per_cu = 0;
     c->tune.graph_steps = 16;
+    c->tune.fused = 0;
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -449,6 +550,7 @@ extern "C" void ms_destroy(ms_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     destroy_graph(c);
+    free_tiles(c);
     for (void* p : c->allocs) hipFree(p);
     if (c->d_tri_chunks) hipFree(c->d_tri_chunks);
     if (c->d_node_chunks) hipFree(c->d_node_chunks);
@@ -809,15 +911,7 @@ extern "C" int ms_step(ms_ctx* c, int64_t nsteps, int64_t equil_every, double to
         }
         while (plain > 0) {
             if (c->timing) {
-                const bool mixed = c->precision == MS_MIXED;
-                hipEventRecord(c->ev[0], c->stream);
-                launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells,
-                                  mixed, true);
-                hipEventRecord(c->ev[1], c->stream);
-                StepFlags fl{0, 0, 0.0, 0.0};
-                launch_node_step(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks,
-                                 c->d_shells, c->num_shells, c->d_clock, mixed, fl);
-                hipEventRecord(c->ev[2], c->stream);
+                step_once(c, false, 0, 0.0, 0.0, true);
                 HIPC(c, hipEventSynchronize(c->ev[2]));
                 float a = 0, b = 0;
                 hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
@@ -1002,13 +1096,24 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
     if (t->tri_grid_per_cu >= 0) n.tri_grid_per_cu = t->tri_grid_per_cu;
     if (t->node_grid_per_cu >= 0) n.node_grid_per_cu = t->node_grid_per_cu;
     if (t->graph_steps > 0) n.graph_steps = t->graph_steps;
+    if (t->fused == 0 || t->fused == 1)
+        n.fused = t->fused;
+    else if (t->fused != -1)
+        return fail(c, MS_ERR_ARG, "fused must be 0, 1 or -1");
     const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
     if (int r = pull_shells(c)) return r;
+    const ms_tuning old = c->tune;
     c->tune = n;
     destroy_graph(c);
     if (rechunk) {
         if (int r = build_chunks(c)) return r;
         if (int r = push_shells(c)) return r;
+    }
+    if (n.fused == 1 && !c->tiles_ready) {
+        if (int r = build_tiles(c)) {
+            c->tune.fused = old.fused;  // keep a working configuration
+            return r;
+        }
     }
     return MS_OK;
 }

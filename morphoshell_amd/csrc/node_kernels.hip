@@ -26,6 +26,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "node_math.h"
 
 namespace ms {
 namespace {
@@ -137,61 +138,9 @@ struct NodeKArgs {
 template <typename CT>
 __device__ __forceinline__ void gather(const NodeData& nd, const CT* contrib, int n, double& fx, double& fy,
                                        double& fz) {
-    const long long e0 = nd.csr_off[n], e1 = nd.csr_off[n + 1];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    long long e = e0;
-    // four entries' loads in flight at once; the sums keep CSR order
-    for (; e + 4 <= e1; e += 4) {
-        const CT* c0 = contrib + (size_t)nd.csr_ent[e] * 3;
-        const CT* c1 = contrib + (size_t)nd.csr_ent[e + 1] * 3;
-        const CT* c2 = contrib + (size_t)nd.csr_ent[e + 2] * 3;
-        const CT* c3 = contrib + (size_t)nd.csr_ent[e + 3] * 3;
-        const double x0 = c0[0], y0 = c0[1], z0 = c0[2], x1 = c1[0], y1 = c1[1], z1 = c1[2];
-        const double x2 = c2[0], y2 = c2[1], z2 = c2[2], x3 = c3[0], y3 = c3[1], z3 = c3[2];
-        ax = ax + x0;
-        ay = ay + y0;
-        az = az + z0;
-        ax = ax + x1;
-        ay = ay + y1;
-        az = az + z1;
-        ax = ax + x2;
-        ay = ay + y2;
-        az = az + z2;
-        ax = ax + x3;
-        ay = ay + y3;
-        az = az + z3;
-    }
-    for (; e < e1; ++e) {
-        const CT* c = contrib + (size_t)nd.csr_ent[e] * 3;
-        ax = ax + (double)c[0];
-        ay = ay + (double)c[1];
-        az = az + (double)c[2];
-    }
-    fx = ax;
-    fy = ay;
-    fz = az;
+    const int* ent = nd.csr_ent;
+    ordered_sum3(contrib, [ent](long long e) { return ent[e]; }, nd.csr_off[n], nd.csr_off[n + 1], fx, fy, fz);
 }
-
-// Damping term of one dof, evaluated as the reference's statement.
-__device__ __forceinline__ double damped(double f, double eta, double m, double v, double den) {
-    double t = -eta * m;
-    t = t * v;
-    t = t / den;
-    return f + t;
-}
-
-// The update of one dof (advance_dynamics.cpp:81-92).
-__device__ __forceinline__ void update(const ShellDev& s, double m, double f, double& v, double& x) {
-    if (!s.gd) {
-        v = v + (s.dt / m) * f;
-    } else {
-        v = v + (s.dt / m) * f;
-        v = (s.damp_den * f) / (s.eta * m);
-    }
-    x = x + s.dt * v;
-}
-
-__device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
 
 // Per-step bookkeeping, one small launch after a step-like node launch (the
 // kernel boundary makes the node launch's partials and flags visible, so no
@@ -552,6 +501,16 @@ void launch_node_step(const Launch& L, const NodeData& nd, const Buffers& b, con
         hipLaunchKernelGGL((bookkeeping_kernel<0, true>), dim3(1), dim3(256), 0, L.stream, k);
     else
         hipLaunchKernelGGL((bookkeeping_kernel<0, false>), dim3(1), dim3(256), 0, L.stream, k);
+}
+
+void launch_step_bookkeeping(hipStream_t s, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
+                             ShellDev* shells, int num_shells, ClockDev* clock, const StepFlags& fl) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, num_shells, clock);
+    k.fl = fl;
+    if (fl.check_equil)
+        hipLaunchKernelGGL((bookkeeping_kernel<0, true>), dim3(1), dim3(256), 0, s, k);
+    else
+        hipLaunchKernelGGL((bookkeeping_kernel<0, false>), dim3(1), dim3(256), 0, s, k);
 }
 
 void launch_node_gather(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,

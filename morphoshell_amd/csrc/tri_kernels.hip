@@ -1,119 +1,20 @@
-// Per-triangle kernels of the per-step path (gfx950).
+// Per-triangle launches of the per-step path (gfx950).
 //
 // One thread per triangle, one chunk of triangles (of one shell) per block.
-// Per triangle the kernel streams its patch record (6 node ids, G, H, dial
-// info, h, mu, area), gathers its six patch nodes' positions, forms the
-// surface derivatives R_x, R_y (G applied to the 3 vertices) and R_xx, R_xy,
-// R_yy (H applied to the 6 patch nodes), the metric a, the normal n and the
-// curvature b, the stretch and bend energy densities and their gradient, and
-// writes the 18 force contributions -A dW/dx of its six patch nodes.  These
-// are gathered per node, in a fixed order, by node_kernels.hip: no atomics.
-//
-// The energy and its derivatives are the reference's (calc_energy_densities
-// .cpp:62-77, calc_deformation_forces.cpp:93-171), restated by the chain
-// rule through the 2x2 quadratic form Phi(D) = nu tr(PD)^2 + (1-nu) tr((PD)^2),
-// P = adj(abar):
-//   W_s = h mu/(1-nu) det(abar)^-3/2 Phi(a - abar)
-//   W_b = W_b0 (1 + W_b0/(gamma mu h)),  W_b0 = mu h^3/(12(1-nu)) det^-3/2 Phi(b - bbar)
-//   dW/dR_x = 2 W_a0 R_x + W_a1 R_y + R_y x u,  dW/dR_y = W_a1 R_x + 2 W_a2 R_y + u x R_x
-//   dW/dR_k'' = -W_bk n,  u = (I - n n^T) w / |R_x x R_y|,  w = -sum_k W_bk R_k''
-// where W_ak = dW_s/da_k and W_bk = dW_b/db_k (stored-component gradients).
-//
-// FP64 mode: everything in double, absolute coordinates (the reference's
-// q = M x dataflow).  Mixed mode: edge vectors to vertex 0 and the strains
-// a - abar, b - bbar are formed in double, the patch fit, stresses and
-// contributions in float, contributions stored as float.
+// The force launch streams each triangle's patch record (6 node ids, G, H,
+// dial info, h, mu, area), gathers its six patch nodes and writes the 18
+// force contributions -A dW/dx of its patch nodes (tri_math.h); they are
+// gathered per node, in a fixed order, by node_kernels.hip: no atomics.
+// The output-time launches evaluate the energy densities (with per-chunk
+// compensated partial sums) and the geometry a, b, n.
 
 #include <math.h>
 
 #include "kernels.h"
+#include "tri_math.h"
 
 namespace ms {
 namespace {
-
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous run of chunks so neighbouring patches share an L2.
-__device__ __forceinline__ int xcd_remap(int i, int n) {
-    const int X = 8;
-    const int q = n / X, r = n % X;
-    const int j = i % X, local = i / X;
-    return j * q + (j < r ? j : r) + local;
-}
-
-// Factor the programmed metric is evaluated at: under the device-run protocol
-// the last dialled one; otherwise the dial clock's.
-__device__ __forceinline__ double dial_factor(const ShellDev& s) {
-    return s.auto_protocol ? s.f_targets : shell_dial_factor(s);
-}
-
-// Programmed abar, bbar of triangle t at dial factor f (do_dialling.cpp:83-84
-// protocol 1, 104-124 protocol 3, 194-228 blend from the ansatz a, b).
-__device__ __forceinline__ void targets(const TriData& td, const ShellDev& s, double f, int t, double ab[3],
-                                        double bb[3]) {
-    const int T = td.T;
-    double A0[3] = {1.0, 0.0, 1.0}, B0[3] = {0.0, 0.0, 0.0};
-    if (s.has_a0) {
-        A0[0] = td.a0[t];
-        A0[1] = td.a0[T + t];
-        A0[2] = td.a0[2 * T + t];
-    }
-    if (s.has_b0) {
-        B0[0] = td.b0[t];
-        B0[1] = td.b0[T + t];
-        B0[2] = td.b0[2 * T + t];
-    }
-    const double fin = s.from_ansatz ? 1.0 : f;
-    if (s.protocol == 3) {
-        const double c2 = td.ainfo[t], cs = td.ainfo[T + t], s2 = td.ainfo[2 * T + t];
-        const double lamf = td.ainfo[3 * T + t], m2nu = td.ainfo[4 * T + t];
-        const double lam = (1.0 - fin) * 1.0 + fin * lamf;
-        const double l2 = lam * lam, lp = pow(lam, m2nu);
-        ab[0] = l2 * c2 + lp * s2;
-        ab[1] = (l2 - lp) * cs;
-        ab[2] = lp * c2 + l2 * s2;
-        bb[0] = bb[1] = bb[2] = 0.0;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            ab[k] = (1.0 - fin) * A0[k] + fin * td.ainfo[k * T + t];
-            bb[k] = (1.0 - fin) * B0[k] + fin * td.binfo[k * T + t];
-        }
-    }
-    if (s.from_ansatz) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            ab[k] = (1.0 - f) * A0[k] + f * ab[k];
-            bb[k] = (1.0 - f) * B0[k] + f * bb[k];
-        }
-    }
-}
-
-// Phi(D) and its stored-component gradient (dPhi/dD00, dPhi/dD01 counted
-// twice, dPhi/dD11) for symmetric D and P given as (xx, xy, yy).
-template <typename C>
-__device__ __forceinline__ C phi_grad(C nu, const C P[3], const C D[3], C g[3]) {
-    const C m00 = P[0] * D[0] + P[1] * D[1], m01 = P[0] * D[1] + P[1] * D[2];
-    const C m10 = P[1] * D[0] + P[2] * D[1], m11 = P[1] * D[1] + P[2] * D[2];
-    const C tr = m00 + m11;
-    const C one_m_nu = C(1) - nu;
-    // P D P
-    const C q00 = m00 * P[0] + m01 * P[1];
-    const C q01 = m00 * P[1] + m01 * P[2];
-    const C q11 = m10 * P[1] + m11 * P[2];
-    const C two_nu_tr = C(2) * nu * tr;
-    g[0] = two_nu_tr * P[0] + C(2) * one_m_nu * q00;
-    g[1] = C(2) * (two_nu_tr * P[1] + C(2) * one_m_nu * q01);
-    g[2] = two_nu_tr * P[2] + C(2) * one_m_nu * q11;
-    return nu * tr * tr + one_m_nu * (m00 * m00 + C(2) * m01 * m10 + m11 * m11);
-}
-
-template <typename C>
-__device__ __forceinline__ C phi_only(C nu, const C P[3], const C D[3]) {
-    const C m00 = P[0] * D[0] + P[1] * D[1], m01 = P[0] * D[1] + P[1] * D[2];
-    const C m10 = P[1] * D[0] + P[2] * D[1], m11 = P[1] * D[1] + P[2] * D[2];
-    const C tr = m00 + m11;
-    return nu * tr * tr + (C(1) - nu) * (m00 * m00 + C(2) * m01 * m10 + m11 * m11);
-}
 
 // ---- compensated block reduction (two-sum, fixed tree: deterministic) ----
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
@@ -180,76 +81,68 @@ struct TriKArgs {
     double* gn;
 };
 
-// ------------------------------------------------------------- FP64 forces
-template <int MODE>
-__global__ __launch_bounds__(256) void tri_fp64_kernel(TriKArgs k) {
-    const TriData& td = k.td;
-    const int T = td.T, N = td.N;
+// ------------------------------------------------------------------ forces
+template <bool MIXED>
+__global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
         const ShellDev& sh = k.shells[ch.z];
         if (k.skip_halted && sh.halted) continue;  // block-uniform
         const int t = ch.x + threadIdx.x;
-        const bool live = t < ch.y;
+        if (t >= ch.y) continue;
+        const double* X = sh.par ? k.pos1 : k.pos;
+        if (MIXED) {
+            float out[18];
+            tri_forces_mixed(k.td, sh, t, X, out);
+            float2* dst = reinterpret_cast<float2*>(k.contrib32 + (size_t)t * 18);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
+        } else {
+            double out[18];
+            tri_forces_fp64(k.td, sh, t, X, out);
+            double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);
+        }
+    }
+}
+
+// ---------------------------------------------- energies and geometry (FP64)
+template <int MODE>
+__global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
+    const TriData& td = k.td;
+    const int T = td.T, N = td.N;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const int ci = xcd_remap(vi, k.nchunks);
+        const Chunk ch = k.chunks[ci];
+        const ShellDev& sh = k.shells[ch.z];
+        const int t = ch.x + threadIdx.x;
         double Es = 0.0, Eb = 0.0;
-        if (live) {
+        if (t < ch.y) {
             const double* X = sh.par ? k.pos1 : k.pos;
-            int p[6];
-#pragma unroll
-            for (int s = 0; s < 6; ++s) p[s] = td.pid[s * T + t];
-            double Px[6], Py[6], Pz[6];
-#pragma unroll
+            double Rx[3] = {0, 0, 0}, Ry[3] = {0, 0, 0}, R2[3][3] = {};
             for (int s = 0; s < 6; ++s) {
-                Px[s] = X[p[s]];
-                Py[s] = X[N + p[s]];
-                Pz[s] = X[2 * N + p[s]];
-            }
-            double G0[3], G1[3], H[6][3];
-#pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                G0[v] = td.G[v * T + t];
-                G1[v] = td.G[(3 + v) * T + t];
-            }
-#pragma unroll
-            for (int s = 0; s < 6; ++s)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) H[s][c] = td.H[(s * 3 + c) * T + t];
-            double Rx[3], Ry[3], R2[3][3];  // R2[k][c]: k = xx, xy, yy
-            Rx[0] = G0[0] * Px[0] + G0[1] * Px[1] + G0[2] * Px[2];
-            Rx[1] = G0[0] * Py[0] + G0[1] * Py[1] + G0[2] * Py[2];
-            Rx[2] = G0[0] * Pz[0] + G0[1] * Pz[1] + G0[2] * Pz[2];
-            Ry[0] = G1[0] * Px[0] + G1[1] * Px[1] + G1[2] * Px[2];
-            Ry[1] = G1[0] * Py[0] + G1[1] * Py[1] + G1[2] * Py[2];
-            Ry[2] = G1[0] * Pz[0] + G1[1] * Pz[1] + G1[2] * Pz[2];
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {
-                double sx = 0.0, sy = 0.0, sz = 0.0;
-#pragma unroll
-                for (int s = 0; s < 6; ++s) {
-                    sx += H[s][kk] * Px[s];
-                    sy += H[s][kk] * Py[s];
-                    sz += H[s][kk] * Pz[s];
+                const int p = td.pid[s * T + t];
+                const double P[3] = {X[p], X[N + p], X[2 * N + p]};
+                for (int c = 0; c < 3; ++c) {
+                    if (s < 3) {
+                        Rx[c] += td.G[s * T + t] * P[c];
+                        Ry[c] += td.G[(3 + s) * T + t] * P[c];
+                    }
+                    for (int kk = 0; kk < 3; ++kk) R2[kk][c] += td.H[(s * 3 + kk) * T + t] * P[c];
                 }
-                R2[kk][0] = sx;
-                R2[kk][1] = sy;
-                R2[kk][2] = sz;
             }
-            // geometry (calc_a_comps_and_b_comps_and_normals.cpp:49-60)
-            const double a0 = Rx[0] * Rx[0] + Rx[1] * Rx[1] + Rx[2] * Rx[2];
-            const double a1 = Rx[0] * Ry[0] + Rx[1] * Ry[1] + Rx[2] * Ry[2];
-            const double a2 = Ry[0] * Ry[0] + Ry[1] * Ry[1] + Ry[2] * Ry[2];
+            const double a[3] = {Rx[0] * Rx[0] + Rx[1] * Rx[1] + Rx[2] * Rx[2],
+                                 Rx[0] * Ry[0] + Rx[1] * Ry[1] + Rx[2] * Ry[2],
+                                 Ry[0] * Ry[0] + Ry[1] * Ry[1] + Ry[2] * Ry[2]};
             const double m0 = Rx[1] * Ry[2] - Rx[2] * Ry[1];
             const double m1 = Rx[2] * Ry[0] - Rx[0] * Ry[2];
             const double m2 = Rx[0] * Ry[1] - Rx[1] * Ry[0];
-            const double mn = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
-            const double imn = 1.0 / mn;
+            const double imn = 1.0 / sqrt(m0 * m0 + m1 * m1 + m2 * m2);
             const double n[3] = {m0 * imn, m1 * imn, m2 * imn};
             double b[3];
-#pragma unroll
             for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
             if (MODE == TRI_GEOMETRY) {
-                const double a[3] = {a0, a1, a2};
-#pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
                     k.ga[kk * T + t] = a[kk];
                     k.gb[kk * T + t] = b[kk];
@@ -264,188 +157,29 @@ __global__ __launch_bounds__(256) void tri_fp64_kernel(TriKArgs k) {
                 const double h = td.h[t], mu = td.mu[t], nu = sh.nu;
                 const double cs = h * mu / (1.0 - nu) * idet32;
                 const double cb = mu * h * h * h / (12.0 * (1.0 - nu)) * idet32;
-                const double Da[3] = {a0 - ab[0], a1 - ab[1], a2 - ab[2]};
+                const double Da[3] = {a[0] - ab[0], a[1] - ab[1], a[2] - ab[2]};
                 const double Db[3] = {b[0] - bb[0], b[1] - bb[1], b[2] - bb[2]};
-                const double scale = sh.gamma * mu * h;
-                const double A = td.area[t];
-                if (MODE == TRI_ENERGY) {
-                    const double Ws = cs * phi_only(nu, P, Da);
-                    const double Wb0 = cb * phi_only(nu, P, Db);
-                    const double Wb = Wb0 * (1.0 + Wb0 / scale);
-                    if (k.dens) {
-                        k.dens[t] = Ws;
-                        k.dens[T + t] = Wb;
-                    }
-                    Es = A * Ws;
-                    Eb = A * Wb;
-                } else {
-                    double ga[3], gbv[3];
-                    phi_grad(nu, P, Da, ga);
-                    const double Wb0 = cb * phi_grad(nu, P, Db, gbv);
-                    const double stiff = 1.0 + 2.0 * Wb0 / scale;  // calc_deformation_forces.cpp:141
-                    const double Wa[3] = {cs * ga[0], cs * ga[1], cs * ga[2]};
-                    const double cbs = cb * stiff;
-                    const double Wb[3] = {cbs * gbv[0], cbs * gbv[1], cbs * gbv[2]};
-                    double w[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) w[c] = -(Wb[0] * R2[0][c] + Wb[1] * R2[1][c] + Wb[2] * R2[2][c]);
-                    const double nw = n[0] * w[0] + n[1] * w[1] + n[2] * w[2];
-                    double u[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) u[c] = (w[c] - nw * n[c]) * imn;
-                    const double cx[3] = {Ry[1] * u[2] - Ry[2] * u[1], Ry[2] * u[0] - Ry[0] * u[2],
-                                          Ry[0] * u[1] - Ry[1] * u[0]};
-                    const double cy[3] = {u[1] * Rx[2] - u[2] * Rx[1], u[2] * Rx[0] - u[0] * Rx[2],
-                                          u[0] * Rx[1] - u[1] * Rx[0]};
-                    double Agx[3], Agy[3], An[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        Agx[c] = A * (2.0 * Wa[0] * Rx[c] + Wa[1] * Ry[c] + cx[c]);
-                        Agy[c] = A * (Wa[1] * Rx[c] + 2.0 * Wa[2] * Ry[c] + cy[c]);
-                        An[c] = A * n[c];
-                    }
-                    double out[18];
-#pragma unroll
-                    for (int s = 0; s < 6; ++s) {
-                        const double ws = H[s][0] * Wb[0] + H[s][1] * Wb[1] + H[s][2] * Wb[2];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            double f = ws * An[c];
-                            if (s < 3) f -= G0[s] * Agx[c] + G1[s] * Agy[c];
-                            out[3 * s + c] = f;
-                        }
-                    }
-                    double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);
+                const double Ws = cs * phi_only(nu, P, Da);
+                const double Wb0 = cb * phi_only(nu, P, Db);
+                const double Wb = Wb0 * (1.0 + Wb0 / (sh.gamma * mu * h));
+                if (k.dens) {
+                    k.dens[t] = Ws;
+                    k.dens[T + t] = Wb;
                 }
+                Es = td.area[t] * Ws;
+                Eb = td.area[t] * Wb;
             }
         }
         if (MODE == TRI_ENERGY) {
             double hi[2] = {Es, Eb}, lo[2] = {0.0, 0.0};
             block_dd_reduce<2>(hi, lo);
             if (threadIdx.x == 0) {
-                const int c = xcd_remap(vi, k.nchunks);
-                k.part[4 * c + 0] = hi[0];
-                k.part[4 * c + 1] = lo[0];
-                k.part[4 * c + 2] = hi[1];
-                k.part[4 * c + 3] = lo[1];
+                k.part[4 * ci + 0] = hi[0];
+                k.part[4 * ci + 1] = lo[0];
+                k.part[4 * ci + 2] = hi[1];
+                k.part[4 * ci + 3] = lo[1];
             }
         }
-    }
-}
-
-// ------------------------------------------------------------ mixed forces
-__global__ __launch_bounds__(256) void tri_mixed_kernel(TriKArgs k) {
-    const TriData& td = k.td;
-    const int T = td.T, N = td.N;
-    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
-        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
-        const ShellDev& sh = k.shells[ch.z];
-        if (k.skip_halted && sh.halted) continue;
-        const int t = ch.x + threadIdx.x;
-        if (t >= ch.y) continue;
-        const double* X = sh.par ? k.pos1 : k.pos;
-        int p[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) p[s] = td.pid[s * T + t];
-        // edge vectors to vertex 0, formed in double, then rounded
-        const double x0 = X[p[0]], y0 = X[N + p[0]], z0 = X[2 * N + p[0]];
-        float e[6][3];
-        e[0][0] = e[0][1] = e[0][2] = 0.0f;
-#pragma unroll
-        for (int s = 1; s < 6; ++s) {
-            e[s][0] = (float)(X[p[s]] - x0);
-            e[s][1] = (float)(X[N + p[s]] - y0);
-            e[s][2] = (float)(X[2 * N + p[s]] - z0);
-        }
-        float G0[3], G1[3], H[6][3];
-#pragma unroll
-        for (int v = 0; v < 3; ++v) {
-            G0[v] = td.G32[v * T + t];
-            G1[v] = td.G32[(3 + v) * T + t];
-        }
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) H[s][c] = td.H32[(s * 3 + c) * T + t];
-        float Rx[3], Ry[3], R2[3][3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            Rx[c] = G0[1] * e[1][c] + G0[2] * e[2][c];
-            Ry[c] = G1[1] * e[1][c] + G1[2] * e[2][c];
-        }
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int s = 1; s < 6; ++s) acc += H[s][kk] * e[s][c];
-                R2[kk][c] = acc;
-            }
-        double ab[3], bb[3];
-        targets(td, sh, dial_factor(sh), t, ab, bb);
-        // strains: products of float derivatives are exact in double
-        const double dRx0 = Rx[0], dRx1 = Rx[1], dRx2 = Rx[2], dRy0 = Ry[0], dRy1 = Ry[1], dRy2 = Ry[2];
-        const float Da[3] = {(float)((dRx0 * dRx0 + dRx1 * dRx1 + dRx2 * dRx2) - ab[0]),
-                             (float)((dRx0 * dRy0 + dRx1 * dRy1 + dRx2 * dRy2) - ab[1]),
-                             (float)((dRy0 * dRy0 + dRy1 * dRy1 + dRy2 * dRy2) - ab[2])};
-        const float m0 = Rx[1] * Ry[2] - Rx[2] * Ry[1];
-        const float m1 = Rx[2] * Ry[0] - Rx[0] * Ry[2];
-        const float m2 = Rx[0] * Ry[1] - Rx[1] * Ry[0];
-        const float imn = rsqrtf(m0 * m0 + m1 * m1 + m2 * m2);
-        const float n[3] = {m0 * imn, m1 * imn, m2 * imn};
-        float Db[3];
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-            const float bk = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
-            Db[kk] = (float)((double)bk - bb[kk]);
-        }
-        const double det = ab[0] * ab[2] - ab[1] * ab[1];
-        const double idet32 = 1.0 / (det * sqrt(det));
-        const float h = td.hmuA32[t], mu = td.hmuA32[T + t], A = td.hmuA32[2 * T + t];
-        const float nu = (float)sh.nu;
-        const float cs = (float)((double)h * (double)mu / (1.0 - sh.nu) * idet32);
-        const float cb = (float)((double)mu * h * h * h / (12.0 * (1.0 - sh.nu)) * idet32);
-        const float P[3] = {(float)ab[2], (float)-ab[1], (float)ab[0]};
-        float ga[3], gbv[3];
-        phi_grad(nu, P, Da, ga);
-        const float Wb0 = cb * phi_grad(nu, P, Db, gbv);
-        const float stiff = 1.0f + 2.0f * Wb0 / (float)(sh.gamma * (double)mu * (double)h);
-        const float Wa[3] = {cs * ga[0], cs * ga[1], cs * ga[2]};
-        const float cbs = cb * stiff;
-        const float Wb[3] = {cbs * gbv[0], cbs * gbv[1], cbs * gbv[2]};
-        float w[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) w[c] = -(Wb[0] * R2[0][c] + Wb[1] * R2[1][c] + Wb[2] * R2[2][c]);
-        const float nw = n[0] * w[0] + n[1] * w[1] + n[2] * w[2];
-        float u[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[c] = (w[c] - nw * n[c]) * imn;
-        const float cx[3] = {Ry[1] * u[2] - Ry[2] * u[1], Ry[2] * u[0] - Ry[0] * u[2], Ry[0] * u[1] - Ry[1] * u[0]};
-        const float cy[3] = {u[1] * Rx[2] - u[2] * Rx[1], u[2] * Rx[0] - u[0] * Rx[2], u[0] * Rx[1] - u[1] * Rx[0]};
-        float Agx[3], Agy[3], An[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            Agx[c] = A * (2.0f * Wa[0] * Rx[c] + Wa[1] * Ry[c] + cx[c]);
-            Agy[c] = A * (Wa[1] * Rx[c] + 2.0f * Wa[2] * Ry[c] + cy[c]);
-            An[c] = A * n[c];
-        }
-        float out[18];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            const float ws = H[s][0] * Wb[0] + H[s][1] * Wb[1] + H[s][2] * Wb[2];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float f = ws * An[c];
-                if (s < 3) f -= G0[s] * Agx[c] + G1[s] * Agy[c];
-                out[3 * s + c] = f;
-            }
-        }
-        float2* dst = reinterpret_cast<float2*>(k.contrib32 + (size_t)t * 18);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
     }
 }
 
@@ -470,16 +204,16 @@ void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, con
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.skip_halted = skip_halted ? 1 : 0;
     if (mixed)
-        hipLaunchKernelGGL(tri_mixed_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+        hipLaunchKernelGGL(tri_forces_kernel<true>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
     else
-        hipLaunchKernelGGL(tri_fp64_kernel<TRI_FORCES>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+        hipLaunchKernelGGL(tri_forces_kernel<false>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
                        ShellDev* shells, double* dens) {
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.dens = dens;
-    hipLaunchKernelGGL(tri_fp64_kernel<TRI_ENERGY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    hipLaunchKernelGGL(tri_output_kernel<TRI_ENERGY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
@@ -488,7 +222,7 @@ void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, c
     k.ga = ga;
     k.gb = gb;
     k.gn = gn;
-    hipLaunchKernelGGL(tri_fp64_kernel<TRI_GEOMETRY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    hipLaunchKernelGGL(tri_output_kernel<TRI_GEOMETRY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 
 }  // namespace ms

@@ -251,7 +251,7 @@ def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000):
 
 
 def relax_batch(shells, dial_steps: int, tol: float, max_steps: int, starts=None, dial_start: float = 0.0,
-                precision: str = "mixed", poll: int = 5000, equil_interval: int = 100):
+                precision: str = "fp64", poll: int = 5000, equil_interval: int = 100):
     """Relax a batch with the device-run protocol; returns (reports, end dofs, seconds)."""
     from .shell import Context
     from .workloads import flat_noisy_state
@@ -276,7 +276,11 @@ def generate(num: int, seed: int, batch: int = 128, dial_steps: int = 2000, tol:
         shells = training_family(min(batch, num - b0), seed + b0)
         reps, ends, secs = relax_batch(shells, dial_steps, tol, max_steps)
         ok = [i for i, r in enumerate(reps) if r["status"] == 2]
-        log(f"batch {b0 // batch}: {len(ok)}/{len(shells)} equilibria in {secs:.1f}s")
+        st = np.bincount([r["status"] for r in reps], minlength=3)
+        fails = sum(r["failed"] for r in reps)
+        fr = np.array([r["max_force_ratio"] for r in reps])
+        log(f"batch {b0 // batch}: {len(ok)}/{len(shells)} equilibria in {secs:.1f}s (status counts {st.tolist()}, "
+            f"{fails} blow-ups, force ratio median {np.median(fr):.2e} max {fr.max():.2e})")
         data += [(shells[i], ends[i]) for i in ok]
     return data
 

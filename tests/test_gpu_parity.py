@@ -312,3 +312,67 @@ def test_fused_equilibrium_check_and_halt():
         ctx.step(1)
         plain.step(1)
         assert np.array_equal(ctx.get_state(0)[0], plain.get_state(0)[0])
+
+
+# ------------------------------------------------- fused owner-computes step
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("node_block,grid", [(64, 0), (128, 2), (256, 0)])
+def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid):
+    """The fused tile launch sums the same contributions in the same
+    patch-incidence order as the two-launch path: states agree bit for bit,
+    including the equilibrium maxima of checked steps."""
+    x, v = deformed_state(sh, seed=40)
+    with _ctx(sh, precision) as a, _ctx(sh, precision) as b:
+        a.set_state(0, x, v)
+        b.set_state(0, x, v)
+        a.set_tuning(node_block=node_block, node_grid_per_cu=grid)
+        b.set_tuning(node_block=node_block, node_grid_per_cu=grid, fused=1)
+        assert b.get_tuning()["fused"] == 1
+        ra = a.step(41, equil_every=10, force_tol=1e-30, speed_tol=1e-30)
+        rb = b.step(41, equil_every=10, force_tol=1e-30, speed_tol=1e-30)
+        xa, va = a.get_state(0)
+        xb, vb = b.get_state(0)
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        for k in ("max_force_ratio", "max_speed_ratio", "max_force_node", "max_speed_node", "checked"):
+            assert ra[0][k] == rb[0][k], k
+        b.enable_timing(True)
+        b.step(3)
+        a.step(3)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+
+
+def test_fused_tile_step_batch_matches_oracle():
+    shells = [_shell(0.08, seed=1), _shell(0.05, seed=2, kappa=-1.0, clamp_every=7), _shell(0.11, seed=3, h=0.02)]
+    states = [deformed_state(s, seed=50 + i) for i, s in enumerate(shells)]
+    with _ctx(shells) as ctx, _ctx(shells) as ref:
+        ctx.set_tuning(node_block=128, fused=1)
+        for i, st in enumerate(states):
+            ctx.set_state(i, *st)
+            ref.set_state(i, *st)
+        ctx.step(30)
+        ref.step(30)
+        for i, s in enumerate(shells):
+            xg, vg = ctx.get_state(i)
+            assert np.array_equal(xg, ref.get_state(i)[0]) and np.array_equal(vg, ref.get_state(i)[1])
+            xo, vo = states[i][0].copy(), states[i][1].copy()
+            assert orc.System.from_shell(s).run_steps(xo, vo, 30) == -1
+            assert np.abs(xg - xo).max() <= 1e-11
+
+
+def test_fused_tile_step_reports_blowup(sh):
+    from morphoshell_amd.shell import HighForceError
+
+    x, v = deformed_state(sh, seed=13)
+    N = sh.N
+    free = np.where(sh.mesh.constrained == 0)[0]
+    x[free[40]] += 0.5
+    x[2 * N + free[300]] += 0.5
+    sysm = orc.System.from_shell(sh)
+    d_o, bad_o = sysm.advance(x.copy(), v.copy(), sysm.full_forces(x, v))
+    with _ctx(sh) as ctx:
+        ctx.set_tuning(fused=1)
+        ctx.set_state(0, x, v)
+        with pytest.raises(HighForceError) as ei:
+            ctx.step(4)
+        assert ei.value.dof == d_o
+        assert np.array_equal(ctx.get_state(0)[0], x)
