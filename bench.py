@@ -71,13 +71,42 @@ def kernel_times(ctx, states, steps: int, strict: bool = True) -> np.ndarray:
 
 
 def kernel_bytes(T: int, N: int, mixed: bool) -> list[int]:
-    """Algorithmic bytes per launch.  Triangle kernel: the per-triangle record
-    (6 patch ids 24 B + G, H 192 B FP64 / 96 B FP32 + abar/bbar dial info 48 B
-    + h, mu, area 24 B / 12 B) and each node position once (24 B).  Node
-    kernel: x, v read and written (96 B) and the mass (8 B).  The per-slot
-    contributions passed between the two launches are intermediates."""
+    """Algorithmic bytes per launch of the two-launch form.  Triangle kernel:
+    the per-triangle record (6 patch ids 24 B + G, H 192 B FP64 / 96 B FP32 +
+    abar/bbar dial info 48 B + h, mu, area 24 B / 12 B) and each node position
+    once (24 B).  Node kernel: x, v read and written (96 B) and the mass (8 B).
+    The per-slot contributions passed between the two launches are
+    intermediates."""
     rec = 180 if mixed else 288
     return [T * rec + N * 24, N * 104]
+
+
+def step_bytes(T: int, N: int, mixed: bool) -> int:
+    """Algorithmic bytes of one step: every triangle record once (288 B FP64 /
+    180 B mixed) and per node x, v read and written plus the mass (104 B).
+    The fused tile kernel does the whole step in one launch, so this is also
+    its per-launch figure; halo triangles that neighbouring tiles read again
+    are not counted."""
+    return T * (180 if mixed else 288) + N * 104
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_traffic_grf_disc_1.8M.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), or
+    None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("TRAFFIC_JSON "):
+                    d = json.loads(line[len("TRAFFIC_JSON "):])
+                    if kernel in d:
+                        return dict(d[kernel], source=os.path.relpath(TRAFFIC_PROFILE, ROOT))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(sh, x, v, budget_s: float) -> dict:
@@ -117,19 +146,24 @@ def bench_single(args) -> dict:
     log(f"T={T} N={N} mesh + set-up {t_setup:.1f}s")
     x, v = workloads.flat_noisy_state(sh)
     states = [(x, v)]
+    # the product form is the fused tile step (256-node tiles); the
+    # triangle + node launch form is timed beside it for reference
+    forms = {"fused": dict(node_block=256, fused=1), "two_launch": dict(tri_block=256, node_block=256, fused=0)}
     res = {}
-    for prec in ("fp64", "mixed"):
+    for prec in args.precisions.split(","):
         with Context(sh, precision=prec) as ctx:
-            secs = time_steps(ctx, states, args.steps, args.warmup)
-            kt = kernel_times(ctx, states, min(args.steps, 50))
-            res[prec] = {"secs": secs, "kt": kt}
-            log(f"{prec}: {args.steps} steps {secs * 1e3:.2f} ms -> {T * args.steps / secs / 1e9:.3f} G tri-evals/s;"
-                f" kernel ms tri {kt[0]:.4f} node {kt[1]:.4f}")
+            for form in args.forms.split(","):
+                ctx.set_tuning(tri_grid_per_cu=0, node_grid_per_cu=0, **forms[form])
+                secs = time_steps(ctx, states, args.steps, args.warmup)
+                kt = kernel_times(ctx, states, min(args.steps, 50))
+                res[(prec, form)] = {"secs": secs, "kt": kt}
+                log(f"{prec} {form}: {args.steps} steps {secs * 1e3:.2f} ms -> "
+                    f"{T * args.steps / secs / 1e9:.3f} G tri-evals/s; kernel ms {kt[0]:.4f} + {kt[1]:.4f}")
     if args.calibrate:
         cal_ms = np.zeros(1)
         native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
-        res["copy_GBs"] = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
-        log(f"copy kernel 1 GiB: {cal_ms[0]:.3f} ms = {res['copy_GBs']:.0f} GB/s")
+        copy_gbs = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
+        log(f"copy kernel 1 GiB: {cal_ms[0]:.3f} ms = {copy_gbs:.0f} GB/s")
 
     parity = {}
     if not args.no_parity:
@@ -146,6 +180,16 @@ def bench_single(args) -> dict:
                 E = ctx.calc_energies(0)
         Eo = sysm.energies(x, v)
         parity["fp64_energy_rel_err"] = float(np.abs(E[:2] - Eo[:2]).max() / max(abs(Eo[:2]).max(), 1e-300))
+        # a fused-form trajectory against the oracle's
+        n = 20
+        with Context(sh, precision="fp64") as ctx:
+            ctx.set_tuning(node_block=256, fused=1)
+            ctx.set_state(0, x, v)
+            ctx.step(n)
+            xg = ctx.get_state(0)[0]
+        xo, vo = x.copy(), v.copy()
+        sysm.run_steps(xo, vo, n)
+        parity[f"fp64_fused_{n}_step_position_max_abs_err"] = float(np.abs(xg - xo).max())
         log(f"parity {parity}")
 
     cpu = None
@@ -154,17 +198,39 @@ def bench_single(args) -> dict:
         cpu = cpu_baseline(sh, x, v, args.cpu_budget)
         log(f"cpu {cpu['value'] / 1e6:.2f} M tri-evals/s on {cpu['cores']} threads")
 
-    r64 = res["fp64"]
-    value = T * args.steps / r64["secs"]
-    ms_step = r64["secs"] / args.steps * 1e3
-    kt = r64["kt"]
-    dominant = 0 if kt[0] >= kt[1] else 1
-    kb = kernel_bytes(T, N, False)
-    names = ["tri_fp64_kernel (per-triangle forces)", "node_step_kernel (gather + dynamics)"]
-    achieved = kb[dominant] / (kt[dominant] * 1e-3) / 1e9
-    step_bytes = T * 288 + N * 104
-    rm = res["mixed"]
-    kbm = kernel_bytes(T, N, True)
+    def summary(prec, form):
+        r = res.get((prec, form))
+        if r is None:
+            return None
+        mixed = prec == "mixed"
+        ms = r["secs"] / args.steps * 1e3
+        kt = r["kt"]
+        d = {"value": T * args.steps / r["secs"], "ms_per_step": ms,
+             "step_alg_GBs": step_bytes(T, N, mixed) / (ms * 1e-3) / 1e9}
+        if form == "fused":
+            d["kernel_ms"] = {"tile_step": kt[0], "bookkeeping": kt[1]}
+        else:
+            kb = kernel_bytes(T, N, mixed)
+            d["kernel_ms"] = {"tri": kt[0], "node": kt[1]}
+            d["tri_alg_GBs"] = kb[0] / (kt[0] * 1e-3) / 1e9
+        return d
+
+    main_prec = "fp64" if any(p == "fp64" for p, _ in res) else "mixed"
+    main_form = "fused" if (main_prec, "fused") in res else "two_launch"
+    r0 = res[(main_prec, main_form)]
+    mixed0 = main_prec == "mixed"
+    value = T * args.steps / r0["secs"]
+    ms_step = r0["secs"] / args.steps * 1e3
+    kt = r0["kt"]
+    if main_form == "fused":
+        kname = f"tile_step_kernel<{'mixed' if mixed0 else 'fp64'}> (fused forces + gather + dynamics)"
+        kbytes, kms = step_bytes(T, N, mixed0), kt[0]
+    else:
+        dom = 0 if kt[0] >= kt[1] else 1
+        kname = ["tri_forces_kernel", "node_step_kernel"][dom]
+        kbytes, kms = kernel_bytes(T, N, mixed0)[dom], kt[dom]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = measured_traffic(f"tile_step_kernel_{main_prec}") if main_form == "fused" else None
     out = {
         "metric": "triangle force-evals/s",
         "value": value,
@@ -176,46 +242,36 @@ def bench_single(args) -> dict:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if not mixed0 else "f32 forces / f64 state",
         "data": "synthetic",
         "config": {
             "workload": f"single disc shell, spacing {args.spacing}, {T} triangles, {N} nodes, GRF lambda metric "
-                        "(amp 0.1, corr 0.2, 64 modes), h=0.005, nu=0.5, FP64 forces and integration",
+                        "(amp 0.1, corr 0.2, 64 modes), h=0.005, nu=0.5, FP64 forces and integration, "
+                        "fused tile step",
             "num_tris": T,
             "num_nodes": N,
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": names[dominant],
+            "kernel": kname,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "alg_bytes_per_launch": kb[dominant],
-            "avg_launch_ms": kt[dominant],
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_detail": traffic,
+            "alg_bytes_per_launch": kbytes,
+            "alg_bytes_per_triangle": kbytes / T,
+            "avg_launch_ms": kms,
         },
-        "step_roofline": {
-            "alg_bytes_per_step": step_bytes,
-            "achieved": step_bytes / (ms_step * 1e-3) / 1e9,
-            "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "kernel_ms": {"tri": kt[0], "node": kt[1]},
-        },
-        "mixed": {
-            "value": T * args.steps / rm["secs"],
-            "ms_per_step": rm["secs"] / args.steps * 1e3,
-            "dtype": "fp32 forces / f64 state",
-            "kernel_ms": {"tri": rm["kt"][0], "node": rm["kt"][1]},
-            "tri_achieved_GBs": kbm[0] / (rm["kt"][0] * 1e-3) / 1e9,
-            "step_alg_GBs": (T * 180 + N * 104) / (rm["secs"] / args.steps) / 1e9,
-        },
+        "forms": {f"{p}_{f}": summary(p, f) for (p, f) in res},
         "parity": parity,
         "cpu_baseline": cpu,
         "setup_s": t_setup,
     }
-    if "copy_GBs" in res:
-        out["copy_kernel_GBs"] = res["copy_GBs"]
+    if args.calibrate:
+        out["copy_kernel_GBs"] = copy_gbs
     return out
 
 
@@ -266,9 +322,10 @@ def bench_batch(args) -> dict:
                                "per-shell GRF lambda, bbar = kappa I, h in [0.002, 0.02]",
                    "num_shells": len(shells), "num_tris": T, "num_nodes": N},
         "autotune": tuned,
-        "kernel_ms": {"tri": kt[0], "node": kt[1]},
-        "tri_alg_GBs": kb[0] / (kt[0] * 1e-3) / 1e9,
-        "step_alg_GBs": (T * 180 + N * 104) / (secs / args.steps) / 1e9,
+        "kernel_ms": ({"tile_step": kt[0], "bookkeeping": kt[1]} if tuned.get("fused") else
+                      {"tri": kt[0], "node": kt[1]}),
+        "dominant_alg_GBs": (step_bytes(T, N, True) if tuned.get("fused") else kb[0]) / (kt[0] * 1e-3) / 1e9,
+        "step_alg_GBs": step_bytes(T, N, True) / (secs / args.steps) / 1e9,
         "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
                   "shells_failed": failed, "shells_per_s": done / t_relax if t_relax > 0 else 0.0,
                   "tolerances": {"force": 1e-6, "speed": 1e-5}},
@@ -369,6 +426,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="also run the copy kernel used to scale counters")
+    ap.add_argument("--precisions", default="fp64,mixed", help="single workload: precisions to time")
+    ap.add_argument("--forms", default="fused,two_launch", help="single workload: step forms to time")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

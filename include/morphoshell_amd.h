@@ -322,7 +322,7 @@ typedef struct ms_tuning {
                                  tile is a node chunk; its workgroup recomputes
                                  the halo triangles and gathers from LDS);
                                  0 = triangle launch + node launch; -1 keeps.
-                                 Both give bit-identical states. */
+                                 All give bit-identical states. */
     int32_t reserved[2];
 } ms_tuning;
 int ms_set_tuning(ms_ctx* ctx, const ms_tuning* t);

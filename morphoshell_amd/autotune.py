@@ -26,7 +26,7 @@ DEFAULT_SPACE = {
     "node_block": (128, 256),
     "grid_per_cu": (0, 4, 8),
     "graph_steps": (16,),
-    "fused_node_block": (64, 128, 256),  # tile sizes of the fused step
+    "fused_node_block": (128, 256),  # tile sizes of the fused step
 }
 
 

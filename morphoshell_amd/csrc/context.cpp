@@ -58,7 +58,13 @@ struct ms_ctx {
     int* tile_tris = nullptr;
     unsigned short* tile_dst = nullptr;
     int tile_max_entries = 0;
+    int tile_T = 0;                // total tile entries (triangles counted per tile)
     bool tiles_ready = false;
+    std::vector<void*> tile_allocs;  // tile-ordered record copies
+    int64_t tile_bytes = 0;          // metadata
+    int64_t tile_rec_bytes = 0;      // record copies
+    TriData tile_td{};
+    int acols = 3;
 
     // graphs
     hipGraphExec_t graph_exec = nullptr;
@@ -134,14 +140,64 @@ int build_chunks(ms_ctx* c) {
     return MS_OK;
 }
 
+void free_tile_records(ms_ctx* c) {
+    for (void* p : c->tile_allocs) hipFree(p);
+    c->tile_allocs.clear();
+    c->tile_td = TriData{};
+    c->dev_bytes -= c->tile_rec_bytes;
+    c->tile_rec_bytes = 0;
+}
+
 void free_tiles(ms_ctx* c) {
+    free_tile_records(c);
     if (c->tile_off) hipFree(c->tile_off);
     if (c->tile_tris) hipFree(c->tile_tris);
     if (c->tile_dst) hipFree(c->tile_dst);
     c->tile_off = c->tile_tris = nullptr;
     c->tile_dst = nullptr;
     c->tile_max_entries = 0;
+    c->tile_T = 0;
+    c->dev_bytes -= c->tile_bytes;
+    c->tile_bytes = 0;
     c->tiles_ready = false;
+}
+
+// Tile-ordered copies of every per-triangle stream (gathered on the device).
+int build_tile_records(ms_ctx* c) {
+    free_tile_records(c);
+    const int T = c->T, Tp = c->tile_T;
+    TriData& o = c->tile_td;
+    o.T = Tp;
+    o.N = c->N;
+    auto copy = [&](const void* in, int elem_bytes, int ncomp, const void** out) -> hipError_t {
+        *out = nullptr;
+        if (!in) return hipSuccess;
+        void* p = nullptr;
+        const size_t nb = (size_t)elem_bytes * ncomp * (size_t)std::max(1, Tp);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) return e;
+        c->tile_allocs.push_back(p);
+        c->tile_rec_bytes += (int64_t)nb;
+        c->dev_bytes += (int64_t)nb;
+        launch_tile_gather(c->stream, in, p, elem_bytes, ncomp, T, Tp, c->tile_tris);
+        *out = p;
+        return hipGetLastError();
+    };
+    HIPC(c, copy(c->pid, 4, 6, (const void**)&o.pid));
+    HIPC(c, copy(c->G, 8, 6, (const void**)&o.G));
+    HIPC(c, copy(c->H, 8, 18, (const void**)&o.H));
+    HIPC(c, copy(c->G32, 4, 6, (const void**)&o.G32));
+    HIPC(c, copy(c->H32, 4, 18, (const void**)&o.H32));
+    HIPC(c, copy(c->ainfo, 8, c->acols, (const void**)&o.ainfo));
+    HIPC(c, copy(c->binfo, 8, 3, (const void**)&o.binfo));
+    HIPC(c, copy(c->a0, 8, 3, (const void**)&o.a0));
+    HIPC(c, copy(c->b0, 8, 3, (const void**)&o.b0));
+    HIPC(c, copy(c->h, 8, 1, (const void**)&o.h));
+    HIPC(c, copy(c->mu, 8, 1, (const void**)&o.mu));
+    HIPC(c, copy(c->area, 8, 1, (const void**)&o.area));
+    HIPC(c, copy(c->hmuA32, 4, 3, (const void**)&o.hmuA32));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MS_OK;
 }
 
 // Tile metadata of the fused step for the current node chunks: per tile the
@@ -199,12 +255,17 @@ int build_tiles(ms_ctx* c) {
     HIPC(c, hipMalloc((void**)&c->tile_off, toff.size() * sizeof(int)));
     HIPC(c, hipMalloc((void**)&c->tile_tris, std::max<size_t>(1, tris.size()) * sizeof(int)));
     HIPC(c, hipMalloc((void**)&c->tile_dst, std::max<size_t>(1, dst.size()) * sizeof(unsigned short)));
+    c->tile_bytes = (int64_t)(toff.size() * sizeof(int) + std::max<size_t>(1, tris.size()) * sizeof(int) +
+                              std::max<size_t>(1, dst.size()) * sizeof(unsigned short));
+    c->dev_bytes += c->tile_bytes;
     HIPC(c, hipMemcpy(c->tile_off, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!tris.empty()) {
         HIPC(c, hipMemcpy(c->tile_tris, tris.data(), tris.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPC(c, hipMemcpy(c->tile_dst, dst.data(), dst.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
     }
     c->tile_max_entries = max_entries;
+    c->tile_T = (int)tris.size();
+    if (int r = build_tile_records(c)) return r;
     c->tiles_ready = true;
     return MS_OK;
 }
@@ -279,7 +340,7 @@ void params_to_dev(const ms_shell_params& p, ShellDev& S) {
 }
 
 TileData tile_data(const ms_ctx* c) {
-    return TileData{c->tile_off, c->tile_tris, c->tile_dst, c->tile_max_entries};
+    return TileData{c->tile_off, c->tile_tris, c->tile_dst, c->tile_max_entries, c->tile_td};
 }
 
 // One step; with the fused tuning the tile launch replaces the triangle and
@@ -291,8 +352,8 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
-        launch_tile_step(node_launch(c), td, nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks,
-                         c->d_shells, mixed, fl);
+        launch_tile_step(node_launch(c), nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks, c->d_shells,
+                         mixed, fl);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                                 c->d_clock, fl);
@@ -383,6 +444,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     c->tune.node_grid_per_cu = 0;
     c->tune.graph_steps = 16;
     c->tune.fused = 0;
+    c->acols = acols;
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -696,6 +758,8 @@ extern "C" int ms_set_zero_dial_geometry(ms_ctx* c, int32_t s, const double* a0,
                               sizeof(double) * S.num_tris, hipMemcpyHostToDevice));
         S.has_b0 = 1;
     }
+    if (c->tiles_ready)  // the tile-ordered copies of a0, b0
+        if (int r = build_tile_records(c)) return r;
     destroy_graph(c);  // kernel arguments changed
     return push_shells(c);
 }

@@ -98,24 +98,31 @@ void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* s
 // Plain 16-byte-vector copy of nbytes (traffic-counter calibration).
 void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
 
-// Fused owner-computes step.  Tile c = node chunk c.  tris[tri_off[c] ..
-// tri_off[c+1]) are the triangles (ascending) whose patches touch the tile's
-// nodes; dst[6j + slot] is the position, relative to csr_off[first node of
-// the tile], of the patch-incidence entry (tris[j], slot), or 0xFFFF when
-// that patch node lies outside the tile.
+// Fused owner-computes step.  Tile c = node chunk c.  Entries j in
+// [tri_off[c], tri_off[c+1]) are the triangles (ascending) whose patches touch
+// the tile's nodes: tris[j] is the triangle and td holds a copy of its
+// per-triangle record at index j (halo triangles appear in several tiles), so
+// every tile streams its records contiguously.  dst[6j + slot] is the
+// position, relative to csr_off[first node of the tile], of the
+// patch-incidence entry (tris[j], slot), or 0xFFFF when that patch node lies
+// outside the tile.
 struct TileData {
     const int* tri_off;            // nchunks + 1
     const int* tris;
     const unsigned short* dst;
     int max_entries;               // max over tiles of the tile's CSR entries
+    TriData td;                    // tile-ordered records, td.T = tri_off[nchunks]
 };
+// out[k*Tp + j] = in[k*T + tris[j]] for k < ncomp, j < Tp (elem_bytes 4 or 8).
+void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp,
+                        const int* tris);
 int tile_lds_bytes(int max_entries, bool mixed);
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.
 hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // One fused step of every shell not halted (the work of launch_tri_forces +
 // launch_node_step's node kernel), followed by launch_step_bookkeeping.
-void launch_tile_step(const Launch& L, const TriData& td, const NodeData& nd, const Buffers& b, const TileData& tl,
-                      const Chunk* chunks, int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl);
+void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
+                      int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl);
 // The per-step bookkeeping launch of launch_node_step (one block).
 void launch_step_bookkeeping(hipStream_t s, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                              ShellDev* shells, int num_shells, ClockDev* clock, const StepFlags& fl);

@@ -12,6 +12,8 @@
 
 #include <limits.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "node_math.h"
 #include "tri_math.h"
@@ -20,7 +22,6 @@ namespace ms {
 namespace {
 
 struct TileKArgs {
-    TriData td;
     NodeData nd;
     Buffers b;
     TileData tl;
@@ -78,21 +79,29 @@ __global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
-        const Chunk ch = k.chunks[ci];
+        const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
         const ShellDev& sh = k.shells[ch.z];
-        if (sh.halted) continue;  // block-uniform
-        const bool do_check = CHECK || sh.check_due;
-        const int par = sh.par;
-        const double* X = k.b.x[par];
-        // phase A: contributions of the tile's triangles into LDS
-        const int j0 = k.tl.tri_off[ci], j1 = k.tl.tri_off[ci + 1];
+        if (uni(sh.halted)) continue;  // block-uniform
+        const bool do_check = CHECK || uni(sh.check_due);
+        const int par = uni(sh.par);
+        const double* X = par ? k.b.x[1] : k.b.x[0];
+        const TriParams tp = tri_params(sh);
+        // phase A: contributions of the tile's triangles into LDS, from the
+        // tile's contiguous copy of their records
+        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_off[ci + 1]);
         for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
-            const int t = k.tl.tris[j];
+            // opaque to loop strength reduction: the record loads keep one
+            // 32-bit lane offset against scalar stream bases instead of a
+            // 64-bit induction pointer per stream
+            unsigned t = (unsigned)j;
+            asm volatile("" : "+v"(t));
+            unsigned pc[6];
+            load_patch_ids(k.tl.td, t, pc);
             CT out[18];
             if (MIXED)
-                tri_forces_mixed(k.td, sh, t, X, reinterpret_cast<float*>(out));
+                tri_forces_mixed(k.tl.td, tp, t, pc, X, reinterpret_cast<float*>(out));
             else
-                tri_forces_fp64(k.td, sh, t, X, reinterpret_cast<double*>(out));
+                tri_forces_fp64(k.tl.td, tp, t, pc, X, reinterpret_cast<double*>(out));
             const unsigned short* d = k.tl.dst + (size_t)j * 6;
 #pragma unroll
             for (int s = 0; s < 6; ++s) {
@@ -158,33 +167,61 @@ __global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
     }
 }
 
+template <typename E>
+__global__ __launch_bounds__(256) void tile_gather_kernel(const E* __restrict__ in, E* __restrict__ out, int ncomp,
+                                                          int T, int Tp, const int* __restrict__ tris) {
+    const long long total = (long long)ncomp * Tp;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int kc = (int)(i / Tp), j = (int)(i - (long long)kc * Tp);
+        out[i] = in[(size_t)kc * T + tris[j]];
+    }
+}
+
 }  // namespace
+
+void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp,
+                        const int* tris) {
+    const long long total = (long long)ncomp * Tp;
+    if (total <= 0) return;
+    const int grid = (int)std::min<long long>(8192, (total + 255) / 256);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(tile_gather_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)in, (double*)out,
+                           ncomp, T, Tp, tris);
+    else
+        hipLaunchKernelGGL(tile_gather_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)in, (float*)out,
+                           ncomp, T, Tp, tris);
+}
 
 int tile_lds_bytes(int max_entries, bool mixed) { return max_entries * 3 * (mixed ? 4 : 8); }
 
-hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed) {
-    hipError_t e = hipSuccess;
-    if (lds_bytes_fp64 > 65536) {
-        e = hipFuncSetAttribute((const void*)tile_step_kernel<false, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes_fp64);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)tile_step_kernel<false, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes_fp64);
+template <bool MIXED>
+hipError_t raise_lds(int bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    const void* fns[2] = {(const void*)tile_step_kernel<MIXED, false>, (const void*)tile_step_kernel<MIXED, true>};
+    for (const void* f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
     }
-    if (e == hipSuccess && lds_bytes_mixed > 65536) {
-        e = hipFuncSetAttribute((const void*)tile_step_kernel<true, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes_mixed);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)tile_step_kernel<true, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes_mixed);
-    }
-    return e;
+    return hipSuccess;
 }
 
-void launch_tile_step(const Launch& L, const TriData& td, const NodeData& nd, const Buffers& b, const TileData& tl,
-                      const Chunk* chunks, int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl) {
+hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed) {
+    const hipError_t e = raise_lds<false>(lds_bytes_fp64);
+    return e != hipSuccess ? e : raise_lds<true>(lds_bytes_mixed);
+}
+
+template <bool MIXED>
+void launch_tile_variant(const Launch& L, const TileKArgs& k, size_t lds, bool check) {
+    if (check)
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, true>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+    else
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, false>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+}
+
+void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
+                      int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl) {
     TileKArgs k{};
-    k.td = td;
     k.nd = nd;
     k.b = b;
     k.tl = tl;
@@ -192,19 +229,11 @@ void launch_tile_step(const Launch& L, const TriData& td, const NodeData& nd, co
     k.nchunks = nchunks;
     k.shells = shells;
     k.fl = fl;
-    const dim3 g(L.grid), bl(L.block);
     const size_t lds = (size_t)tile_lds_bytes(tl.max_entries, mixed);
-    if (mixed) {
-        if (fl.check_equil)
-            hipLaunchKernelGGL((tile_step_kernel<true, true>), g, bl, lds, L.stream, k);
-        else
-            hipLaunchKernelGGL((tile_step_kernel<true, false>), g, bl, lds, L.stream, k);
-    } else {
-        if (fl.check_equil)
-            hipLaunchKernelGGL((tile_step_kernel<false, true>), g, bl, lds, L.stream, k);
-        else
-            hipLaunchKernelGGL((tile_step_kernel<false, false>), g, bl, lds, L.stream, k);
-    }
+    if (mixed)
+        launch_tile_variant<true>(L, k, lds, fl.check_equil != 0);
+    else
+        launch_tile_variant<false>(L, k, lds, fl.check_equil != 0);
 }
 
 }  // namespace ms

@@ -85,21 +85,25 @@ struct TriKArgs {
 template <bool MIXED>
 __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
-        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const int ci = xcd_remap(vi, k.nchunks);
+        const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
         const ShellDev& sh = k.shells[ch.z];
-        if (k.skip_halted && sh.halted) continue;  // block-uniform
+        if (k.skip_halted && uni(sh.halted)) continue;  // block-uniform
+        const TriParams tp = tri_params(sh);
+        const double* X = uni(sh.par) ? k.pos1 : k.pos;
         const int t = ch.x + threadIdx.x;
         if (t >= ch.y) continue;
-        const double* X = sh.par ? k.pos1 : k.pos;
+        unsigned p[6];
+        load_patch_ids(k.td, (unsigned)t, p);
         if (MIXED) {
             float out[18];
-            tri_forces_mixed(k.td, sh, t, X, out);
+            tri_forces_mixed(k.td, tp, (unsigned)t, p, X, out);
             float2* dst = reinterpret_cast<float2*>(k.contrib32 + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
         } else {
             double out[18];
-            tri_forces_fp64(k.td, sh, t, X, out);
+            tri_forces_fp64(k.td, tp, (unsigned)t, p, X, out);
             double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
                 }
             } else {
                 double ab[3], bb[3];
-                targets(td, sh, dial_factor(sh), t, ab, bb);
+                targets(td, tri_params(sh), (unsigned)t, ab, bb);
                 const double P[3] = {ab[2], -ab[1], ab[0]};
                 const double det = ab[0] * ab[2] - ab[1] * ab[1];
                 const double idet32 = 1.0 / (det * sqrt(det));

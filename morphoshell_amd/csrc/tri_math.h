@@ -31,33 +31,76 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
     return j * q + (j < r ? j : r) + local;
 }
 
+// ---- uniform values and loads ------------------------------------------------
+// A block works on one shell (chunks never straddle shells), so everything
+// read from the shell record, and every stream base, is the same for all
+// lanes.  Broadcasting those values from lane 0 keeps them in scalar
+// registers; loads then take a scalar base plus a 32-bit per-lane byte offset
+// instead of a 64-bit vector address per stream.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename P>
+__device__ __forceinline__ P* uni_ptr(P* p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (P*)(((unsigned long long)hi << 32) | lo);
+}
+// base[i] for a uniform base and i * sizeof(E) < 2^32.
+template <typename E>
+__device__ __forceinline__ E ld(const E* base, unsigned i) {
+    return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(E));
+}
+
+// The shell quantities the per-triangle mechanics needs, block-uniform.
+struct TriParams {
+    double nu, gamma, f;  // Poisson ratio, stiffening scale, dial factor
+    int protocol, from_ansatz, has_a0, has_b0;
+};
+
 // Factor the programmed metric is evaluated at: under the device-run protocol
 // the last dialled one; otherwise the dial clock's.
 __device__ __forceinline__ double dial_factor(const ShellDev& s) {
     return s.auto_protocol ? s.f_targets : shell_dial_factor(s);
 }
 
-// Programmed abar, bbar of triangle t at dial factor f (do_dialling.cpp:83-84
-// protocol 1, 104-124 protocol 3, 194-228 blend from the ansatz a, b).
-__device__ __forceinline__ void targets(const TriData& td, const ShellDev& s, double f, int t, double ab[3],
+__device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
+    TriParams p;
+    p.nu = uni(s.nu);
+    p.gamma = uni(s.gamma);
+    p.f = uni(dial_factor(s));
+    p.protocol = uni(s.protocol);
+    p.from_ansatz = uni(s.from_ansatz);
+    p.has_a0 = uni(s.has_a0);
+    p.has_b0 = uni(s.has_b0);
+    return p;
+}
+
+// Programmed abar, bbar of triangle t (do_dialling.cpp:83-84 protocol 1,
+// 104-124 protocol 3, 194-228 blend from the ansatz a, b).
+__device__ __forceinline__ void targets(const TriData& td, const TriParams& s, unsigned t, double ab[3],
                                         double bb[3]) {
 #pragma clang fp contract(on)
-    const int T = td.T;
+    const size_t T = (size_t)td.T;
+    const double f = s.f;
     double A0[3] = {1.0, 0.0, 1.0}, B0[3] = {0.0, 0.0, 0.0};
     if (s.has_a0) {
-        A0[0] = td.a0[t];
-        A0[1] = td.a0[T + t];
-        A0[2] = td.a0[2 * T + t];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A0[k] = ld(td.a0 + k * T, t);
     }
     if (s.has_b0) {
-        B0[0] = td.b0[t];
-        B0[1] = td.b0[T + t];
-        B0[2] = td.b0[2 * T + t];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B0[k] = ld(td.b0 + k * T, t);
     }
     const double fin = s.from_ansatz ? 1.0 : f;
     if (s.protocol == 3) {
-        const double c2 = td.ainfo[t], cs = td.ainfo[T + t], s2 = td.ainfo[2 * T + t];
-        const double lamf = td.ainfo[3 * T + t], m2nu = td.ainfo[4 * T + t];
+        const double c2 = ld(td.ainfo, t), cs = ld(td.ainfo + T, t), s2 = ld(td.ainfo + 2 * T, t);
+        const double lamf = ld(td.ainfo + 3 * T, t), m2nu = ld(td.ainfo + 4 * T, t);
         const double lam = (1.0 - fin) * 1.0 + fin * lamf;
         const double l2 = lam * lam, lp = pow(lam, m2nu);
         ab[0] = l2 * c2 + lp * s2;
@@ -67,8 +110,8 @@ __device__ __forceinline__ void targets(const TriData& td, const ShellDev& s, do
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            ab[k] = (1.0 - fin) * A0[k] + fin * td.ainfo[k * T + t];
-            bb[k] = (1.0 - fin) * B0[k] + fin * td.binfo[k * T + t];
+            ab[k] = (1.0 - fin) * A0[k] + fin * ld(td.ainfo + k * T, t);
+            bb[k] = (1.0 - fin) * B0[k] + fin * ld(td.binfo + k * T, t);
         }
     }
     if (s.from_ansatz) {
@@ -108,47 +151,48 @@ __device__ __forceinline__ C phi_only(C nu, const C P[3], const C D[3]) {
     return nu * tr * tr + (C(1) - nu) * (m00 * m00 + C(2) * m01 * m10 + m11 * m11);
 }
 
-// FP64 force contributions of triangle t (out[3*slot + c]); X = positions
-// (3N, SoA with stride N).
-__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const ShellDev& sh, int t, const double* X,
-                                                double out[18]) {
-#pragma clang fp contract(on)
-    const int T = td.T, N = td.N;
-    int p[6];
+// The six patch node ids of triangle t (loaded ahead of the mechanics, so a
+// caller can fetch the next triangle's ids while this one computes).
+__device__ __forceinline__ void load_patch_ids(const TriData& td, unsigned t, unsigned p[6]) {
+    const size_t T = (size_t)td.T;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) p[s] = td.pid[s * T + t];
+    for (int s = 0; s < 6; ++s) p[s] = (unsigned)ld(td.pid + s * T, t);
+}
+
+// FP64 force contributions of triangle t (out[3*slot + c]) with patch node
+// ids p; X = positions (3N, SoA with stride N), uniform.
+__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t,
+                                                const unsigned p[6], const double* X, double out[18]) {
+#pragma clang fp contract(on)
+    const size_t T = (size_t)td.T, N = (size_t)td.N;
     double Px[6], Py[6], Pz[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        Px[s] = X[p[s]];
-        Py[s] = X[N + p[s]];
-        Pz[s] = X[2 * N + p[s]];
+        Px[s] = ld(X, p[s]);
+        Py[s] = ld(X + N, p[s]);
+        Pz[s] = ld(X + 2 * N, p[s]);
     }
-    double G0[3], G1[3], H[6][3];
+    double Rx[3] = {0.0, 0.0, 0.0}, Ry[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-        G0[v] = td.G[v * T + t];
-        G1[v] = td.G[(3 + v) * T + t];
+        const double g0 = ld(td.G + v * T, t), g1 = ld(td.G + (3 + v) * T, t);
+        Rx[0] += g0 * Px[v];
+        Rx[1] += g0 * Py[v];
+        Rx[2] += g0 * Pz[v];
+        Ry[0] += g1 * Px[v];
+        Ry[1] += g1 * Py[v];
+        Ry[2] += g1 * Pz[v];
     }
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) H[s][c] = td.H[(s * 3 + c) * T + t];
-    double Rx[3], Ry[3], R2[3][3];  // R2[k][c]: k = xx, xy, yy
-    Rx[0] = G0[0] * Px[0] + G0[1] * Px[1] + G0[2] * Px[2];
-    Rx[1] = G0[0] * Py[0] + G0[1] * Py[1] + G0[2] * Py[2];
-    Rx[2] = G0[0] * Pz[0] + G0[1] * Pz[1] + G0[2] * Pz[2];
-    Ry[0] = G1[0] * Px[0] + G1[1] * Px[1] + G1[2] * Px[2];
-    Ry[1] = G1[0] * Py[0] + G1[1] * Py[1] + G1[2] * Py[2];
-    Ry[2] = G1[0] * Pz[0] + G1[1] * Pz[1] + G1[2] * Pz[2];
+    double R2[3][3];  // R2[k][c]: k = xx, xy, yy
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
         double sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            sx += H[s][kk] * Px[s];
-            sy += H[s][kk] * Py[s];
-            sz += H[s][kk] * Pz[s];
+            const double hv = ld(td.H + (s * 3 + kk) * T, t);
+            sx += hv * Px[s];
+            sy += hv * Py[s];
+            sz += hv * Pz[s];
         }
         R2[kk][0] = sx;
         R2[kk][1] = sy;
@@ -167,17 +211,17 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const ShellDe
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
     double ab[3], bb[3];
-    targets(td, sh, dial_factor(sh), t, ab, bb);
+    targets(td, sp, t, ab, bb);
     const double P[3] = {ab[2], -ab[1], ab[0]};
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
-    const double h = td.h[t], mu = td.mu[t], nu = sh.nu;
+    const double h = ld(td.h, t), mu = ld(td.mu, t), nu = sp.nu;
     const double cs = h * mu / (1.0 - nu) * idet32;
     const double cb = mu * h * h * h / (12.0 * (1.0 - nu)) * idet32;
     const double Da[3] = {a0 - ab[0], a1 - ab[1], a2 - ab[2]};
     const double Db[3] = {b[0] - bb[0], b[1] - bb[1], b[2] - bb[2]};
-    const double scale = sh.gamma * mu * h;
-    const double A = td.area[t];
+    const double scale = sp.gamma * mu * h;
+    const double A = ld(td.area, t);
     double ga[3], gbv[3];
     phi_grad(nu, P, Da, ga);
     const double Wb0 = cb * phi_grad(nu, P, Db, gbv);
@@ -203,11 +247,12 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const ShellDe
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const double ws = H[s][0] * Wb[0] + H[s][1] * Wb[1] + H[s][2] * Wb[2];
+        const double ws = ld(td.H + (s * 3) * T, t) * Wb[0] + ld(td.H + (s * 3 + 1) * T, t) * Wb[1] +
+                          ld(td.H + (s * 3 + 2) * T, t) * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double f = ws * An[c];
-            if (s < 3) f -= G0[s] * Agx[c] + G1[s] * Agy[c];
+            if (s < 3) f -= ld(td.G + s * T, t) * Agx[c] + ld(td.G + (3 + s) * T, t) * Agy[c];
             out[3 * s + c] = f;
         }
     }
@@ -215,49 +260,43 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const ShellDe
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const ShellDev& sh, int t, const double* X,
-                                                 float out[18]) {
+__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t,
+                                                 const unsigned p[6], const double* X, float out[18]) {
 #pragma clang fp contract(on)
-    const int T = td.T, N = td.N;
-    int p[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) p[s] = td.pid[s * T + t];
-    const double x0 = X[p[0]], y0 = X[N + p[0]], z0 = X[2 * N + p[0]];
+    const size_t T = (size_t)td.T, N = (size_t)td.N;
+    const double x0 = ld(X, p[0]), y0 = ld(X + N, p[0]), z0 = ld(X + 2 * N, p[0]);
     float e[6][3];
     e[0][0] = e[0][1] = e[0][2] = 0.0f;
 #pragma unroll
     for (int s = 1; s < 6; ++s) {
-        e[s][0] = (float)(X[p[s]] - x0);
-        e[s][1] = (float)(X[N + p[s]] - y0);
-        e[s][2] = (float)(X[2 * N + p[s]] - z0);
+        e[s][0] = (float)(ld(X, p[s]) - x0);
+        e[s][1] = (float)(ld(X + N, p[s]) - y0);
+        e[s][2] = (float)(ld(X + 2 * N, p[s]) - z0);
     }
-    float G0[3], G1[3], H[6][3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        G0[v] = td.G32[v * T + t];
-        G1[v] = td.G32[(3 + v) * T + t];
-    }
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) H[s][c] = td.H32[(s * 3 + c) * T + t];
     float Rx[3], Ry[3], R2[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Rx[c] = G0[1] * e[1][c] + G0[2] * e[2][c];
-        Ry[c] = G1[1] * e[1][c] + G1[2] * e[2][c];
-    }
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk)
+    {
+        const float g01 = ld(td.G32 + 1 * T, t), g02 = ld(td.G32 + 2 * T, t);
+        const float g11 = ld(td.G32 + 4 * T, t), g12 = ld(td.G32 + 5 * T, t);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int s = 1; s < 6; ++s) acc += H[s][kk] * e[s][c];
-            R2[kk][c] = acc;
+            Rx[c] = g01 * e[1][c] + g02 * e[2][c];
+            Ry[c] = g11 * e[1][c] + g12 * e[2][c];
         }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 1; s < 6; ++s) {
+            const float hv = ld(td.H32 + (s * 3 + kk) * T, t);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += hv * e[s][c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R2[kk][c] = acc[c];
+    }
     double ab[3], bb[3];
-    targets(td, sh, dial_factor(sh), t, ab, bb);
+    targets(td, sp, t, ab, bb);
     // strains: products of float derivatives are exact in double
     const double dRx0 = Rx[0], dRx1 = Rx[1], dRx2 = Rx[2], dRy0 = Ry[0], dRy1 = Ry[1], dRy2 = Ry[2];
     const float Da[3] = {(float)((dRx0 * dRx0 + dRx1 * dRx1 + dRx2 * dRx2) - ab[0]),
@@ -276,15 +315,15 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const ShellD
     }
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
-    const float h = td.hmuA32[t], mu = td.hmuA32[T + t], A = td.hmuA32[2 * T + t];
-    const float nu = (float)sh.nu;
-    const float cs = (float)((double)h * (double)mu / (1.0 - sh.nu) * idet32);
-    const float cb = (float)((double)mu * h * h * h / (12.0 * (1.0 - sh.nu)) * idet32);
+    const float h = ld(td.hmuA32, t), mu = ld(td.hmuA32 + T, t), A = ld(td.hmuA32 + 2 * T, t);
+    const float nu = (float)sp.nu;
+    const float cs = (float)((double)h * (double)mu / (1.0 - sp.nu) * idet32);
+    const float cb = (float)((double)mu * h * h * h / (12.0 * (1.0 - sp.nu)) * idet32);
     const float P[3] = {(float)ab[2], (float)-ab[1], (float)ab[0]};
     float ga[3], gbv[3];
     phi_grad(nu, P, Da, ga);
     const float Wb0 = cb * phi_grad(nu, P, Db, gbv);
-    const float stiff = 1.0f + 2.0f * Wb0 / (float)(sh.gamma * (double)mu * (double)h);
+    const float stiff = 1.0f + 2.0f * Wb0 / (float)(sp.gamma * (double)mu * (double)h);
     const float Wa[3] = {cs * ga[0], cs * ga[1], cs * ga[2]};
     const float cbs = cb * stiff;
     const float Wb[3] = {cbs * gbv[0], cbs * gbv[1], cbs * gbv[2]};
@@ -306,11 +345,12 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const ShellD
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const float ws = H[s][0] * Wb[0] + H[s][1] * Wb[1] + H[s][2] * Wb[2];
+        const float ws = ld(td.H32 + (s * 3) * T, t) * Wb[0] + ld(td.H32 + (s * 3 + 1) * T, t) * Wb[1] +
+                         ld(td.H32 + (s * 3 + 2) * T, t) * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             float f = ws * An[c];
-            if (s < 3) f -= G0[s] * Agx[c] + G1[s] * Agy[c];
+            if (s < 3) f -= ld(td.G32 + s * T, t) * Agx[c] + ld(td.G32 + (3 + s) * T, t) * Agy[c];
             out[3 * s + c] = f;
         }
     }

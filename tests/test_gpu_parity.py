@@ -376,3 +376,34 @@ def test_fused_tile_step_reports_blowup(sh):
             ctx.step(4)
         assert ei.value.dof == d_o
         assert np.array_equal(ctx.get_state(0)[0], x)
+
+
+def test_dial_from_ansatz_matches_oracle_in_both_step_forms():
+    """Protocol-1 blend from the ansatz geometry (do_dialling.cpp:194-228):
+    forces against the oracle, and the fused form (tiles built before the
+    zero-dial geometry arrives, so their record copies are refreshed) against
+    the two-launch form."""
+    s = _shell(0.06, seed=7, dialling_from_ansatz_rather_than_ref_state=True)
+    rng = np.random.default_rng(3)
+    T = s.T
+    a0 = np.concatenate([1 + 0.02 * rng.random(T), 0.01 * rng.standard_normal(T), 1 + 0.02 * rng.random(T)])
+    b0 = 0.2 * rng.standard_normal(3 * T)
+    x, v = deformed_state(s, seed=33)
+    f, K = 0.4, 1000
+    with _ctx(s) as a, _ctx(s) as b:
+        b.set_tuning(node_block=128, fused=1)
+        for c in (a, b):
+            c.set_zero_dial_geometry(0, a0, b0)
+            c.set_state(0, x, v)
+            c.set_dial_clock(0, 0.0, 1.0, K, int(f * K), True)
+        a.calc_deformation_forces()
+        Fg = a.get_forces(0)
+        a.set_state(0, x, v)
+        a.step(12)
+        b.step(12)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+    ab, bb = orc.dial(T, 1, True, f, s.mesh.abar_info, s.mesh.bbar_info, a0, b0)
+    Fo = orc.System.from_shell(s, abar=ab, bbar=bb).deformation_forces(x)
+    err = np.abs(Fg - Fo).max() / np.abs(Fo).max()
+    print(f"ansatz-dial force rel err {err:.3e}")
+    assert err <= 1e-10
