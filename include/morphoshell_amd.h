@@ -248,6 +248,12 @@ int ms_calc_energies(ms_ctx* ctx, int32_t shell, double out[3]);
 int ms_calc_geometry(ms_ctx* ctx, int32_t shell, double* a, double* b, double* n);
 /* Per-triangle energy densities (T x 2 SoA: stretch.., bend..), may be NULL. */
 int ms_calc_energy_densities(ms_ctx* ctx, int32_t shell, double* densities);
+/* Output-time diagnostics per triangle (T each, any pointer may be NULL):
+ * Gauss and mean curvature of the shape operator (calc_curvatures.cpp:35-94)
+ * and the strain measure sqrt(tr(E^T E)/2), E = (F^+)^T (a - abar) F^+ / 2
+ * (calc_strains.cpp:35-78), both evaluated in their invariant forms
+ * det(b)/det(a), tr(a^-1 b)/2 and sqrt(tr((a^-1 (a - abar))^2)/8). */
+int ms_calc_diagnostics(ms_ctx* ctx, int32_t shell, double* gauss_curv, double* mean_curv, double* strain);
 /* Current force buffer of one shell (3N dofs-SoA). */
 int ms_get_forces(ms_ctx* ctx, int32_t shell, double* forces);
 int ms_set_forces(ms_ctx* ctx, int32_t shell, const double* forces);
@@ -298,6 +304,8 @@ typedef struct ms_relax_report {
     double dial_factor;
     double max_force_ratio;
     double max_speed_ratio;
+    int64_t steps;       /* the shell's own step count since ms_relax_setup (steps
+                            applied to it; a halted shell's count stops) */
 } ms_relax_report;
 
 /* Arm the protocol for one shell (the current state is the start). */

@@ -4,6 +4,8 @@
 
 * ``morphoshell_amd/_lib/libmorphoshell_amd.so``: host set-up (C++/OpenMP),
   the device context and the C-ABI, and the HIP kernels for gfx950;
+* ``morphoshell_amd/_lib/ms_run``: the host driver (settings file and VTK in,
+  VTK out) around the C-ABI;
 * ``oracle/liboracle.so``: the FP64 CPU checker (test infrastructure only).
 
 Every call recompiles (the build is a few seconds of hipcc per file).
@@ -82,6 +84,21 @@ def build_library() -> str:
     return out
 
 
+DRIVER_SOURCES = ["run_settings.cpp", "vtk_io.cpp", "ms_run.cpp"]
+
+
+def build_driver() -> str:
+    """The host driver executable ``_lib/ms_run`` (C++17, links the library)."""
+    d = os.path.join(HERE, "driver")
+    out = os.path.join(LIBDIR, "ms_run")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{d}",
+          *[os.path.join(d, s) for s in DRIVER_SOURCES], f"-L{LIBDIR}", "-lmorphoshell_amd",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> str:
     d = os.path.join(ROOT, "oracle")
     subprocess.run(["make", "-s", "-B", "-C", d], check=True, capture_output=True)
@@ -90,8 +107,10 @@ def build_oracle() -> str:
 
 def main() -> None:
     lib = build_library()
+    drv = build_driver()
     orc = build_oracle()
     print(lib)
+    print(drv)
     print(orc)
 
 

@@ -721,9 +721,6 @@ extern "C" int ms_set_shell_params(ms_ctx* c, int32_t s, const ms_shell_params* 
         S.protocol = old_protocol;
         return fail(c, MS_ERR_ARG, "ms_set_shell_params: the dial protocol is fixed at ms_create");
     }
-    if (S.from_ansatz && !S.has_a0) {
-        // a0 = flat until ms_set_zero_dial_geometry is called
-    }
     return push_shells(c);
 }
 
@@ -911,6 +908,23 @@ extern "C" int ms_calc_geometry(ms_ctx* c, int32_t s, double* a, double* b, doub
             HIPC(c, hipMemcpy(outs[q] + (size_t)k * S.num_tris, srcs[q] + k * T + S.tri_begin,
                               sizeof(double) * S.num_tris, hipMemcpyDeviceToHost));
     }
+    return MS_OK;
+}
+
+extern "C" int ms_calc_diagnostics(ms_ctx* c, int32_t s, double* gauss, double* mean, double* strain) {
+    if (int r = shell_ok(c, s)) return r;
+    if (int r = ensure_scratch(c)) return r;
+    const size_t T = c->T;
+    launch_tri_diagnostics(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells,
+                           c->scratch3);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    double* outs[3] = {gauss, mean, strain};
+    for (int q = 0; q < 3; ++q)
+        if (outs[q])
+            HIPC(c, hipMemcpy(outs[q], c->scratch3 + q * T + S.tri_begin, sizeof(double) * S.num_tris,
+                              hipMemcpyDeviceToHost));
     return MS_OK;
 }
 
@@ -1106,6 +1120,7 @@ extern "C" int ms_relax_run(ms_ctx* c, int64_t max_steps, int64_t poll_every, ms
         R.dial_factor = shell_dial_factor(S);
         R.max_force_ratio = S.max_f;
         R.max_speed_ratio = S.max_v;
+        R.steps = S.own_steps;
     }
     c->err.clear();
     return MS_OK;

@@ -47,7 +47,7 @@ struct Buffers {
     long long* bad_dof;       // per shell, ops-path blow-up scan
 };
 
-enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2 };
+enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2, TRI_DIAGNOSTICS = 3 };
 
 struct Launch {
     int grid;
@@ -64,6 +64,11 @@ void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, con
                        ShellDev* shells, double* dens);
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
                          ShellDev* shells, double* ga, double* gb, double* gn);
+// Output-time diagnostics into out (3T): Gauss curvature det(b)/det(a), mean
+// curvature tr(a^-1 b)/2 and the strain measure sqrt(tr((a^-1 (a-abar))^2)/8)
+// (calc_curvatures.cpp:35-94, calc_strains.cpp:35-78, in invariant form).
+void launch_tri_diagnostics(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                            ShellDev* shells, double* out);
 
 // Node launches.
 struct StepFlags {

@@ -152,6 +152,21 @@ __global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
                     k.gb[kk * T + t] = b[kk];
                     k.gn[kk * T + t] = n[kk];
                 }
+            } else if (MODE == TRI_DIAGNOSTICS) {
+                // the shape operator a^-1 b and the strain a^-1 (a - abar):
+                // invariants of the reference's QR and pseudo-inverse forms
+                double ab[3], bb[3];
+                targets(td, tri_params(sh), (unsigned)t, ab, bb);
+                const double det = a[0] * a[2] - a[1] * a[1];
+                const double gauss = (b[0] * b[2] - b[1] * b[1]) / det;
+                const double mean = 0.5 * (a[2] * b[0] - 2.0 * a[1] * b[1] + a[0] * b[2]) / det;
+                const double D0 = a[0] - ab[0], D1 = a[1] - ab[1], D2 = a[2] - ab[2];
+                const double m00 = (a[2] * D0 - a[1] * D1) / det, m01 = (a[2] * D1 - a[1] * D2) / det;
+                const double m10 = (a[0] * D1 - a[1] * D0) / det, m11 = (a[0] * D2 - a[1] * D1) / det;
+                const double tr2 = m00 * m00 + 2.0 * m01 * m10 + m11 * m11;
+                k.ga[t] = gauss;
+                k.ga[T + t] = mean;
+                k.ga[2 * T + t] = sqrt(tr2 / 8.0);
             } else {
                 double ab[3], bb[3];
                 targets(td, tri_params(sh), (unsigned)t, ab, bb);
@@ -218,6 +233,13 @@ void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, con
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.dens = dens;
     hipLaunchKernelGGL(tri_output_kernel<TRI_ENERGY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_tri_diagnostics(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                            ShellDev* shells, double* out) {
+    TriKArgs k = make_args(td, b, chunks, nchunks, shells);
+    k.ga = out;
+    hipLaunchKernelGGL(tri_output_kernel<TRI_DIAGNOSTICS>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,

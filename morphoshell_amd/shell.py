@@ -104,11 +104,12 @@ class Settings:
 
 
 def _boolstr(v: str) -> bool:
-    if v in ("true", "1"):
+    # read_settings.cpp:290-300 accepts exactly "true" and "false"
+    if v == "true":
         return True
-    if v in ("false", "0"):
+    if v == "false":
         return False
-    raise ValueError(f"not a boolean: {v!r}")
+    raise ValueError("Error: you passed a std::string to boolstring_to_bool that read neither true nor false.")
 
 
 def read_settings(text: str) -> Settings:
@@ -440,6 +441,14 @@ class Context:
         d = np.empty(2 * T)
         self._ck(self.L.ms_calc_energy_densities(self.h, shell, nat.dptr(d)))
         return d.reshape(2, T)
+
+    def calc_diagnostics(self, shell: int = 0) -> dict:
+        """Gauss and mean curvature and the strain measure per triangle
+        (calc_curvatures.cpp:35-94, calc_strains.cpp:35-78)."""
+        T = self.shells[shell].T
+        g, m, s = np.empty(T), np.empty(T), np.empty(T)
+        self._ck(self.L.ms_calc_diagnostics(self.h, shell, nat.dptr(g), nat.dptr(m), nat.dptr(s)))
+        return {"gauss_curv": g, "mean_curv": m, "strain": s}
 
     # -- fused path
     def step(self, nsteps: int, equil_every: int = 0, force_tol: float = 1e-6, speed_tol: float = 1e-5,

@@ -250,45 +250,91 @@ def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000):
     return out
 
 
-def relax_batch(shells, dial_steps: int, tol: float, max_steps: int, starts=None, dial_start: float = 0.0,
-                precision: str = "fp64", poll: int = 5000, equil_interval: int = 100):
-    """Relax a batch with the device-run protocol; returns (reports, end dofs, seconds)."""
+def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int = 2000, precision: str = "fp64",
+                poll: int = 5000, equil_interval: int = 100):
+    """Relax a batch with the device-run protocol.  plans[i] = (start dofs,
+    dial_start, steps per dial phase) or None for the flat start with noise
+    dialled from 0 over dial_steps.  Returns (reports, end dofs, seconds)."""
     from .shell import Context
     from .workloads import flat_noisy_state
 
     t0 = time.perf_counter()
     with Context(shells, precision=precision) as ctx:
         for i, sh in enumerate(shells):
-            x, v = flat_noisy_state(sh, seed=i) if starts is None or starts[i] is None else (starts[i], None)
+            plan = None if plans is None else plans[i]
+            if plan is None:
+                x, v = flat_noisy_state(sh, seed=i)
+                f0, K = 0.0, dial_steps
+            else:
+                x, v = plan[0], None
+                f0, K = plan[1], plan[2]
             ctx.set_state(i, x, v)
-            ctx.relax_setup(i, steps_per_dial_phase=dial_steps, dial_spacing=1.0, dial_start=dial_start,
+            ctx.relax_setup(i, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=f0,
                             equil_interval=equil_interval, tol_f=tol, tol_v=tol)
         reps = ctx.relax_run(max_steps, poll_every=poll)
         ends = [ctx.get_state(i)[0] for i in range(len(shells))]
     return reps, ends, time.perf_counter() - t0
 
 
+def full_dial_energies(shells, states) -> list[float]:
+    """Elastic energy E_s + E_b of each shell's state at dial factor 1."""
+    from .shell import Context
+
+    out = []
+    with Context(shells, precision="fp64") as ctx:
+        for i, x in enumerate(states):
+            ctx.set_dial_clock(i, 1.0, 1.0, 1, 0, False)
+            ctx.set_state(i, x)
+            e = ctx.calc_energies(i)
+            out.append(float(e[0] + e[1]))
+    return out
+
+
 def generate(num: int, seed: int, batch: int = 128, dial_steps: int = 2000, tol: float = 1e-5,
              max_steps: int = 600000) -> list[tuple]:
-    """Equilibria of the training family produced by the batched solver."""
+    """Equilibria of the training family produced by the batched solver:
+    (shell, end dofs, (family seed, family size, index)) per converged shell."""
     data = []
     for b0 in range(0, num, batch):
-        shells = training_family(min(batch, num - b0), seed + b0)
-        reps, ends, secs = relax_batch(shells, dial_steps, tol, max_steps)
+        nb = min(batch, num - b0)
+        shells = training_family(nb, seed + b0)
+        reps, ends, secs = relax_batch(shells, tol, max_steps, dial_steps=dial_steps)
         ok = [i for i, r in enumerate(reps) if r["status"] == 2]
         st = np.bincount([r["status"] for r in reps], minlength=3)
         fails = sum(r["failed"] for r in reps)
         fr = np.array([r["max_force_ratio"] for r in reps])
         log(f"batch {b0 // batch}: {len(ok)}/{len(shells)} equilibria in {secs:.1f}s (status counts {st.tolist()}, "
             f"{fails} blow-ups, force ratio median {np.median(fr):.2e} max {fr.max():.2e})")
-        data += [(shells[i], ends[i]) for i in ok]
+        data += [(shells[i], ends[i], (seed + b0, nb, i)) for i in ok]
     return data
+
+
+def save_equilibria(path: str, data: list[tuple]) -> None:
+    keys = np.array([d[2] for d in data], np.int64).reshape(-1, 3)
+    sizes = np.array([len(d[1]) for d in data], np.int64)
+    np.savez_compressed(path, keys=keys, split=np.cumsum(sizes)[:-1],
+                        dofs=(np.concatenate([d[1] for d in data]) if data else np.zeros(0)).astype(np.float32))
+
+
+def load_equilibria(path: str) -> list[tuple]:
+    """Inverse of save_equilibria; the shells are rebuilt from their seeds."""
+    z = np.load(path)
+    fams = {}
+    out = []
+    for (sd, nb, i), x in zip(z["keys"], np.split(z["dofs"], z["split"])):
+        key = (int(sd), int(nb))
+        if key not in fams:
+            fams[key] = training_family(key[1], key[0])
+        out.append((fams[key][int(i)], np.asarray(x, np.float64), (key[0], key[1], int(i))))
+    return out
 
 
 def train(net: WarmStartNet, data: list[tuple], epochs: int = 60, graphs_per_step: int = 8, lr: float = 2e-3,
           seed: int = 0) -> list[float]:
-    graphs = [shell_graph(sh) for sh, _ in data]
-    targets = [kabsch_target(sh, g, x) for (sh, x), g in zip(data, graphs)]
+    graphs = [shell_graph(d[0]) for d in data]
+    targets = [kabsch_target(d[0], g, d[1]) for d, g in zip(data, graphs)]
+    zero = float(np.mean(np.concatenate([(t.astype(np.float64) ** 2).ravel() for t in targets])))
+    log(f"{len(data)} equilibria; loss of a zero prediction {zero:.5f}")
     rng = np.random.default_rng(seed)
     hist = []
     steps = epochs * max(1, len(data) // graphs_per_step)
@@ -308,60 +354,56 @@ def train(net: WarmStartNet, data: list[tuple], epochs: int = 60, graphs_per_ste
     return hist
 
 
-def warm_start_states(net: WarmStartNet, shells, energies_of) -> tuple[list, list]:
-    """Predicted start states with the energy fallback.  energies_of(i, dofs)
-    returns the elastic energy of shell i at dial factor 1."""
+def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000) -> tuple[list, list]:
+    """Start plans with the energy fallback: the predicted positions at dial
+    factor 1 where their elastic energy (at dial factor 1) is below that of the
+    flat start with noise, otherwise the flat start dialled as usual."""
     from .workloads import flat_noisy_state
 
-    starts, used = [], []
-    for i, sh in enumerate(shells):
-        pred = net.predict_dofs(sh)
-        flat, _ = flat_noisy_state(sh, seed=i)
-        if energies_of(i, pred) < energies_of(i, flat):
-            starts.append(pred)
-            used.append(True)
-        else:
-            starts.append(flat)
-            used.append(False)
-    return starts, used
+    preds = [net.predict_dofs(sh) for sh in shells]
+    flats = [flat_noisy_state(sh, seed=i)[0] for i, sh in enumerate(shells)]
+    e_pred = full_dial_energies(shells, preds)
+    e_flat = full_dial_energies(shells, flats)
+    plans, used = [], []
+    for i in range(len(shells)):
+        ok = bool(np.isfinite(e_pred[i]) and e_pred[i] < e_flat[i])
+        plans.append((preds[i], 1.0, 1) if ok else None)
+        used.append(ok)
+    return plans, used
 
 
 def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: int = 2000, tol: float = 1e-5,
              max_steps: int = 600000) -> dict:
-    """Steps to equilibrium from the learned warm start (at dial factor 1)
-    against the flat start with noise dialled as usual, on unseen seeds."""
-    from .shell import Context
-
+    """Steps to equilibrium from the learned warm start (at dial factor 1,
+    with the fallback) against the flat start with noise dialled as usual, on
+    unseen seeds; final energies of both at dial factor 1."""
     shells = training_family(num, seed)
-    with Context(shells, precision="fp64") as ctx:
-        def energy(i, dofs):
-            ctx.set_state(i, dofs)
-            e = ctx.calc_energies(i)
-            return e[0] + e[1]
-        starts, used = warm_start_states(net, shells, energy)
-    base, base_end, t_base = relax_batch(shells, dial_steps, tol, max_steps)
-    warm, warm_end, t_warm = relax_batch(shells, 1, tol, max_steps, starts=starts, dial_start=1.0)
-    with Context(shells, precision="fp64") as ctx:
-        e_base, e_warm = [], []
-        for i in range(num):
-            ctx.set_state(i, base_end[i])
-            e = ctx.calc_energies(i)
-            e_base.append(e[0] + e[1])
-            ctx.set_state(i, warm_end[i])
-            e = ctx.calc_energies(i)
-            e_warm.append(e[0] + e[1])
+    t0 = time.perf_counter()
+    plans, used = warm_start_plans(net, shells, dial_steps)
+    t_pred = time.perf_counter() - t0
+    base, base_end, t_base = relax_batch(shells, tol, max_steps, dial_steps=dial_steps)
+    warm, warm_end, t_warm = relax_batch(shells, tol, max_steps, plans=plans, dial_steps=dial_steps)
+    e_base = full_dial_energies(shells, base_end)
+    e_warm = full_dial_energies(shells, warm_end)
     both = [i for i in range(num) if base[i]["status"] == 2 and warm[i]["status"] == 2]
+    pred = [i for i in both if used[i]]
     sb = np.array([base[i]["eq_step"] + 1 for i in both], float)
     sw = np.array([warm[i]["eq_step"] + 1 for i in both], float)
-    de = np.array([abs(e_warm[i] - e_base[i]) / max(abs(e_base[i]), 1e-300) for i in both])
+    sbp = np.array([base[i]["eq_step"] + 1 for i in pred], float)
+    swp = np.array([warm[i]["eq_step"] + 1 for i in pred], float)
+    de = np.array([(e_warm[i] - e_base[i]) / max(abs(e_base[i]), 1e-300) for i in both])
+    f = lambda a: float(a) if np.size(a) else None  # noqa: E731
     return {
-        "shells": num, "both_converged": len(both), "prediction_used": int(sum(used)),
-        "steps_flat_mean": float(sb.mean()) if len(sb) else None,
-        "steps_warm_mean": float(sw.mean()) if len(sw) else None,
-        "steps_ratio_median": float(np.median(sb / sw)) if len(sb) else None,
-        "seconds_flat": t_base, "seconds_warm": t_warm,
-        "final_energy_rel_diff_median": float(np.median(de)) if len(de) else None,
-        "final_energy_rel_diff_max": float(de.max()) if len(de) else None,
+        "shells": num, "prediction_used": int(sum(used)), "both_converged": len(both),
+        "flat_converged": sum(1 for r in base if r["status"] == 2),
+        "warm_converged": sum(1 for r in warm if r["status"] == 2),
+        "steps_flat_mean": f(sb.mean()) if len(sb) else None,
+        "steps_warm_mean": f(sw.mean()) if len(sw) else None,
+        "steps_ratio_median": f(np.median(sb / sw)) if len(sb) else None,
+        "steps_ratio_median_predicted": f(np.median(sbp / swp)) if len(sbp) else None,
+        "seconds_flat": t_base, "seconds_warm": t_warm, "seconds_predict": t_pred,
+        "final_energy_rel_diff_median": f(np.median(de)) if len(de) else None,
+        "final_energy_lower_or_equal_fraction": f(np.mean(de <= 1e-6)) if len(de) else None,
         "tolerance": tol,
     }
 
@@ -373,16 +415,26 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="generate equilibria, train the warm-start network, evaluate")
     ap.add_argument("--num-train", type=int, default=512)
     ap.add_argument("--num-eval", type=int, default=64)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=0, help="0: from --train-steps")
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="warmstart_params.npz")
     ap.add_argument("--max-steps", type=int, default=600000)
+    ap.add_argument("--data", default="", help="npz of generated equilibria: loaded if present, else written")
     args = ap.parse_args(argv)
     t0 = time.time()
-    data = generate(args.num_train, seed=1, max_steps=args.max_steps)
+    if args.data and os.path.exists(args.data):
+        data = load_equilibria(args.data)
+        log(f"loaded {len(data)} equilibria from {args.data}")
+    else:
+        data = generate(args.num_train, seed=1, max_steps=args.max_steps)
+        if args.data:
+            save_equilibria(args.data, data)
     t_gen = time.time() - t0
     net = WarmStartNet(seed=0)
     t0 = time.time()
-    hist = train(net, data, epochs=args.epochs)
+    epochs = args.epochs or max(1, math.ceil(args.train_steps * 8 / max(1, len(data))))
+    hist = train(net, data, epochs=epochs, lr=args.lr)
     t_train = time.time() - t0
     net.save(args.out)
     ev = evaluate(net, args.num_eval, max_steps=args.max_steps)

@@ -407,3 +407,46 @@ def test_dial_from_ansatz_matches_oracle_in_both_step_forms():
     err = np.abs(Fg - Fo).max() / np.abs(Fo).max()
     print(f"ansatz-dial force rel err {err:.3e}")
     assert err <= 1e-10
+
+
+def _reference_form_diagnostics(F, b, a, abar):
+    """calc_curvatures.cpp:52-66 and calc_strains.cpp:52-71 as written there:
+    QR of the 3x2 deformation gradient for the symmetric shape operator, and
+    the pseudo-inverse strain tensor (numpy, per triangle)."""
+    T = F.shape[0]
+    gauss, mean, strain = np.empty(T), np.empty(T), np.empty(T)
+    for t in range(T):
+        R = np.linalg.qr(F[t], mode="r")
+        Si = np.linalg.inv(R[:2, :2])
+        B = np.array([[b[0, t], b[1, t]], [b[1, t], b[2, t]]])
+        S = Si.T @ B @ Si
+        gauss[t] = np.linalg.det(S)
+        mean[t] = 0.5 * np.trace(S)
+        A = np.array([[a[0, t], a[1, t]], [a[1, t], a[2, t]]])
+        Ab = np.array([[abar[0, t], abar[1, t]], [abar[1, t], abar[2, t]]])
+        Fp = np.linalg.inv(F[t].T @ F[t]) @ F[t].T
+        E = 0.5 * (Fp.T @ (A - Ab) @ Fp)
+        strain[t] = np.sqrt(0.5 * np.trace(E.T @ E))
+    return gauss, mean, strain
+
+
+def test_output_diagnostics_match_reference_forms():
+    """Device curvatures and strain against the reference's QR and
+    pseudo-inverse forms, evaluated in numpy on the oracle's geometry."""
+    s = _shell(0.08, seed=9, kappa=0.7)
+    x, v = deformed_state(s, seed=60)
+    sysm = orc.System.from_shell(s)
+    with _ctx(s) as ctx:
+        ctx.set_state(0, x, v)
+        ctx.set_dial_clock(0, 1.0, 1.0, 1, 0, False)
+        d = ctx.calc_diagnostics(0)
+        a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
+    q = sysm.surface_derivs(x).reshape(-1, 3, 5)   # per triangle: X, Y, Z rows of (Rx, Ry, Rxx, Rxy, Ryy)
+    F = q[:, :, 0:2]
+    abar = s.mesh.abar_info.T
+    g_ref, m_ref, s_ref = _reference_form_diagnostics(F, b, a, abar)
+    for name, got, ref in (("gauss", d["gauss_curv"], g_ref), ("mean", d["mean_curv"], m_ref),
+                           ("strain", d["strain"], s_ref)):
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
+        print(f"{name}: rel err {err:.2e}")
+        assert err <= 1e-10, name
