@@ -230,6 +230,19 @@ def lce_defect_abar_info(centroids: np.ndarray, lam_final: float | np.ndarray, n
     return np.stack([ang, lam, np.full_like(ang, nu_ot)], axis=1)
 
 
+def defect_metric_abar_info(centroids: np.ndarray, lam: np.ndarray, nu_ot: float,
+                            offset: float = math.pi / 2) -> np.ndarray:
+    """Protocol-1 abar_info (xx, xy, yy) of a +1 azimuthal LCE defect at the
+    origin with a per-triangle lambda: abar = lambda^2 d d + lambda^(-2 nu) d' d',
+    d the director at angle phi + offset, d' its normal (the dial-factor-1
+    metric of do_dialling.cpp:104-124)."""
+    ang = np.arctan2(centroids[:, 1], centroids[:, 0]) + offset
+    c, s = np.cos(ang), np.sin(ang)
+    lam = np.asarray(lam, dtype=np.float64)
+    l2, lp = lam * lam, lam ** (-2.0 * nu_ot)
+    return np.stack([l2 * c * c + lp * s * s, (l2 - lp) * c * s, lp * c * c + l2 * s * s], axis=1)
+
+
 def uniform_bbar_info(num_tris: int, kappa: float) -> np.ndarray:
     """bbar = kappa I for every triangle."""
     return np.tile([kappa, 0.0, kappa], (num_tris, 1)).astype(np.float64)

@@ -226,25 +226,43 @@ class WarmStartNet:
 
 # ------------------------------------------------------------ data + train
 
-def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000):
-    """Small thick shells that relax in ~1e5 steps: discs/squares, GRF lambda
-    (amplitude 0.05-0.15, clipped at 3 sigma), bbar = kappa I (|kappa| <= 1.5),
-    h in [0.02, 0.05]."""
-    from .meshing import disc_mesh, grf, isotropic_abar_info, square_mesh, uniform_bbar_info
+FAMILIES = ("defect", "grf")
+
+
+def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000, family: str = "defect"):
+    """Small thick shells that relax in ~1e5 steps, h in [0.02, 0.05].
+
+    ``defect``: discs with a +1 azimuthal LCE defect metric (final lambda
+    0.85-0.95 modulated by a clipped GRF of amplitude 0.03, opto-thermal nu
+    0.5) and bbar = kappa I, kappa in [0, 1]: the equilibrium is a cone-like
+    shape fixed by the defect, unique up to the z -> -z reflection.
+    ``grf``: discs/squares with an isotropic GRF lambda (amplitude 0.05-0.15)
+    and bbar = kappa I, |kappa| <= 1.5: thin discs then bend cylindrically
+    about an axis that only the noise picks, so the equilibrium is not a
+    function of the programmed metric."""
+    from .meshing import (defect_metric_abar_info, disc_mesh, grf, isotropic_abar_info, square_mesh,
+                          uniform_bbar_info)
     from .shell import Settings, prepare_shell
 
+    if family not in FAMILIES:
+        raise ValueError(f"unknown family {family!r}")
     out = []
     for i in range(num):
-        rng = np.random.default_rng([seed, i, 7])
+        rng = np.random.default_rng([seed, i, 7 if family == "grf" else 11])
         T = math.exp(rng.uniform(math.log(tmin), math.log(tmax)))
-        if rng.uniform() < 0.5:
+        if family == "defect" or rng.uniform() < 0.5:
             m = disc_mesh(math.sqrt(7.26 / T), seed=seed * 1009 + i)
         else:
             m = square_mesh(math.sqrt(9.24 / T), seed=seed * 1009 + i)
         c = m.centroids()
-        amp = rng.uniform(0.05, 0.15)
-        m.abar_info = isotropic_abar_info(1.0 + amp * np.clip(grf(c[:, :2], 0.3, 64, seed * 7907 + i), -3, 3))
-        m.bbar_info = uniform_bbar_info(m.num_tris, rng.uniform(-1.5, 1.5))
+        g = np.clip(grf(c[:, :2], 0.3, 64, seed * 7907 + i), -3, 3)
+        if family == "defect":
+            lam = rng.uniform(0.85, 0.95) * (1.0 + 0.03 * g)
+            m.abar_info = defect_metric_abar_info(c, lam, 0.5)
+            m.bbar_info = uniform_bbar_info(m.num_tris, rng.uniform(0.0, 1.0))
+        else:
+            m.abar_info = isotropic_abar_info(1.0 + rng.uniform(0.05, 0.15) * g)
+            m.bbar_info = uniform_bbar_info(m.num_tris, rng.uniform(-1.5, 1.5))
         m.fill_defaults(h=rng.uniform(0.02, 0.05))
         out.append(prepare_shell(m, Settings(def_poisson_ratio=0.5)))
     return out
@@ -276,28 +294,33 @@ def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int 
     return reps, ends, time.perf_counter() - t0
 
 
-def full_dial_energies(shells, states) -> list[float]:
-    """Elastic energy E_s + E_b of each shell's state at dial factor 1."""
+def full_dial_energies(shells, states) -> list:
+    """Elastic energy E_s + E_b at dial factor 1 of each shell's state; states[i]
+    may be one dofs vector or a list of them (a list of energies back)."""
     from .shell import Context
 
     out = []
     with Context(shells, precision="fp64") as ctx:
-        for i, x in enumerate(states):
+        for i, xs in enumerate(states):
             ctx.set_dial_clock(i, 1.0, 1.0, 1, 0, False)
-            ctx.set_state(i, x)
-            e = ctx.calc_energies(i)
-            out.append(float(e[0] + e[1]))
+            one = isinstance(xs, np.ndarray)
+            es = []
+            for x in ([xs] if one else xs):
+                ctx.set_state(i, x)
+                e = ctx.calc_energies(i)
+                es.append(float(e[0] + e[1]))
+            out.append(es[0] if one else es)
     return out
 
 
 def generate(num: int, seed: int, batch: int = 128, dial_steps: int = 2000, tol: float = 1e-5,
-             max_steps: int = 600000) -> list[tuple]:
+             max_steps: int = 600000, family: str = "defect") -> list[tuple]:
     """Equilibria of the training family produced by the batched solver:
     (shell, end dofs, (family seed, family size, index)) per converged shell."""
     data = []
     for b0 in range(0, num, batch):
         nb = min(batch, num - b0)
-        shells = training_family(nb, seed + b0)
+        shells = training_family(nb, seed + b0, family=family)
         reps, ends, secs = relax_batch(shells, tol, max_steps, dial_steps=dial_steps)
         ok = [i for i, r in enumerate(reps) if r["status"] == 2]
         st = np.bincount([r["status"] for r in reps], minlength=3)
@@ -309,22 +332,23 @@ def generate(num: int, seed: int, batch: int = 128, dial_steps: int = 2000, tol:
     return data
 
 
-def save_equilibria(path: str, data: list[tuple]) -> None:
+def save_equilibria(path: str, data: list[tuple], family: str) -> None:
     keys = np.array([d[2] for d in data], np.int64).reshape(-1, 3)
     sizes = np.array([len(d[1]) for d in data], np.int64)
-    np.savez_compressed(path, keys=keys, split=np.cumsum(sizes)[:-1],
+    np.savez_compressed(path, keys=keys, split=np.cumsum(sizes)[:-1], family=family,
                         dofs=(np.concatenate([d[1] for d in data]) if data else np.zeros(0)).astype(np.float32))
 
 
 def load_equilibria(path: str) -> list[tuple]:
     """Inverse of save_equilibria; the shells are rebuilt from their seeds."""
     z = np.load(path)
+    family = str(z["family"]) if "family" in z else "grf"
     fams = {}
     out = []
     for (sd, nb, i), x in zip(z["keys"], np.split(z["dofs"], z["split"])):
         key = (int(sd), int(nb))
         if key not in fams:
-            fams[key] = training_family(key[1], key[0])
+            fams[key] = training_family(key[1], key[0], family=family)
         out.append((fams[key][int(i)], np.asarray(x, np.float64), (key[0], key[1], int(i))))
     return out
 
@@ -354,30 +378,43 @@ def train(net: WarmStartNet, data: list[tuple], epochs: int = 60, graphs_per_ste
     return hist
 
 
+# scales of the predicted displacement tried before the energy test: the
+# network's errors cost stretch energy, so a shorter step along the
+# prediction often beats both the flat start and the full prediction
+ALPHAS = (1.0, 0.75, 0.5, 0.35, 0.25, 0.15)
+
+
 def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000) -> tuple[list, list]:
-    """Start plans with the energy fallback: the predicted positions at dial
-    factor 1 where their elastic energy (at dial factor 1) is below that of the
-    flat start with noise, otherwise the flat start dialled as usual."""
+    """Start plans with the energy fallback.  For each shell the predicted
+    displacement is scaled by the alpha of least elastic energy (at dial
+    factor 1); the scaled prediction starts at dial factor 1 when its energy
+    is below that of the flat start with noise, otherwise the flat start is
+    dialled as usual.  Returns (plans, chosen alpha or None)."""
     from .workloads import flat_noisy_state
 
-    preds = [net.predict_dofs(sh) for sh in shells]
     flats = [flat_noisy_state(sh, seed=i)[0] for i, sh in enumerate(shells)]
-    e_pred = full_dial_energies(shells, preds)
+    cands = []
+    for sh in shells:
+        pred = net.predict_dofs(sh)
+        ref = sh.ref_pos
+        cands.append([ref + a * (pred - ref) for a in ALPHAS])
+    e_pred = full_dial_energies(shells, cands)
     e_flat = full_dial_energies(shells, flats)
     plans, used = [], []
     for i in range(len(shells)):
-        ok = bool(np.isfinite(e_pred[i]) and e_pred[i] < e_flat[i])
-        plans.append((preds[i], 1.0, 1) if ok else None)
-        used.append(ok)
+        k = int(np.nanargmin(e_pred[i])) if np.isfinite(e_pred[i]).any() else -1
+        ok = k >= 0 and e_pred[i][k] < e_flat[i]
+        plans.append((cands[i][k], 1.0, 1) if ok else None)
+        used.append(ALPHAS[k] if ok else None)
     return plans, used
 
 
 def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: int = 2000, tol: float = 1e-5,
-             max_steps: int = 600000) -> dict:
+             max_steps: int = 600000, family: str = "defect") -> dict:
     """Steps to equilibrium from the learned warm start (at dial factor 1,
     with the fallback) against the flat start with noise dialled as usual, on
     unseen seeds; final energies of both at dial factor 1."""
-    shells = training_family(num, seed)
+    shells = training_family(num, seed, family=family)
     t0 = time.perf_counter()
     plans, used = warm_start_plans(net, shells, dial_steps)
     t_pred = time.perf_counter() - t0
@@ -386,7 +423,7 @@ def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: i
     e_base = full_dial_energies(shells, base_end)
     e_warm = full_dial_energies(shells, warm_end)
     both = [i for i in range(num) if base[i]["status"] == 2 and warm[i]["status"] == 2]
-    pred = [i for i in both if used[i]]
+    pred = [i for i in both if used[i] is not None]
     sb = np.array([base[i]["eq_step"] + 1 for i in both], float)
     sw = np.array([warm[i]["eq_step"] + 1 for i in both], float)
     sbp = np.array([base[i]["eq_step"] + 1 for i in pred], float)
@@ -394,7 +431,8 @@ def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: i
     de = np.array([(e_warm[i] - e_base[i]) / max(abs(e_base[i]), 1e-300) for i in both])
     f = lambda a: float(a) if np.size(a) else None  # noqa: E731
     return {
-        "shells": num, "prediction_used": int(sum(used)), "both_converged": len(both),
+        "shells": num, "prediction_used": sum(u is not None for u in used),
+        "alphas_chosen": [u for u in used if u is not None], "both_converged": len(both),
         "flat_converged": sum(1 for r in base if r["status"] == 2),
         "warm_converged": sum(1 for r in warm if r["status"] == 2),
         "steps_flat_mean": f(sb.mean()) if len(sb) else None,
@@ -421,15 +459,16 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default="warmstart_params.npz")
     ap.add_argument("--max-steps", type=int, default=600000)
     ap.add_argument("--data", default="", help="npz of generated equilibria: loaded if present, else written")
+    ap.add_argument("--family", default="defect", choices=FAMILIES)
     args = ap.parse_args(argv)
     t0 = time.time()
     if args.data and os.path.exists(args.data):
         data = load_equilibria(args.data)
         log(f"loaded {len(data)} equilibria from {args.data}")
     else:
-        data = generate(args.num_train, seed=1, max_steps=args.max_steps)
+        data = generate(args.num_train, seed=1, max_steps=args.max_steps, family=args.family)
         if args.data:
-            save_equilibria(args.data, data)
+            save_equilibria(args.data, data, args.family)
     t_gen = time.time() - t0
     net = WarmStartNet(seed=0)
     t0 = time.time()
@@ -437,7 +476,7 @@ def main(argv=None) -> int:
     hist = train(net, data, epochs=epochs, lr=args.lr)
     t_train = time.time() - t0
     net.save(args.out)
-    ev = evaluate(net, args.num_eval, max_steps=args.max_steps)
+    ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family)
     print(json.dumps({"equilibria": len(data), "generate_s": t_gen, "train_s": t_train,
                       "loss_first": hist[0], "loss_last": hist[-1], "eval": ev}), flush=True)
     return 0
