@@ -90,7 +90,7 @@ def step_bytes(T: int, N: int, mixed: bool) -> int:
     return T * (180 if mixed else 288) + N * 104
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_traffic_grf_disc_1.8M.txt")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_grf_disc_1.8M.txt")
 
 
 def measured_traffic(kernel: str):

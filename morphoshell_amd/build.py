@@ -92,7 +92,7 @@ def build_driver() -> str:
     d = os.path.join(HERE, "driver")
     out = os.path.join(LIBDIR, "ms_run")
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{d}",
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", f"-I{os.path.join(ROOT, 'include')}", f"-I{d}",
           *[os.path.join(d, s) for s in DRIVER_SOURCES], f"-L{LIBDIR}", "-lmorphoshell_amd",
           "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, out)

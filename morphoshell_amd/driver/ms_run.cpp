@@ -11,7 +11,9 @@
 // whole dialling / waiting / equilibrium state machine of main.cpp:269-507
 // runs on the GPU (ms_relax_setup / ms_relax_run), pausing at every
 // equilibrium so this program can write its output, and stopping at a
-// regular output step to write that one.
+// regular output step to write that one.  The output's device data are
+// copied off synchronously; formatting and writing the file happen on a
+// writer thread while the device steps on.
 //
 // Differences, each deliberate: the run stops at the first equilibrium at
 // dial factor 1 (the reference's loop has no end unless glass slides are
@@ -24,6 +26,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <ctime>
 #include <filesystem>
@@ -114,6 +122,71 @@ std::string make_output_dir(const std::string& settings, const std::string& ref,
     return dir.string();
 }
 
+// One output's data, copied off the device; formatted and written by the
+// writer thread while the device keeps stepping.
+struct OutputJob {
+    int64_t stepcount;
+    double dial_factor;
+    int status;
+    std::vector<double> dofs, gauss, mean, strain, dens;
+};
+
+// Single background thread writing stepcount_<n>_output.vtk files (and the
+// aux_output.txt rows) in submission order.
+class OutputWriter {
+  public:
+    using Fn = std::function<void(const OutputJob&)>;
+    explicit OutputWriter(Fn fn) : fn_(std::move(fn)), th_([this] { loop(); }) {}
+    ~OutputWriter() { finish(); }
+    void submit(OutputJob job) {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (!error_.empty()) throw std::runtime_error(error_);
+        q_.push_back(std::move(job));
+        cv_.notify_all();
+    }
+    // Wait for every submitted job; rethrows the first write error.
+    void finish() {
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            done_ = true;
+            cv_.notify_all();
+        }
+        if (th_.joinable()) th_.join();
+        if (!error_.empty()) {
+            const std::string e = error_;
+            error_.clear();
+            throw std::runtime_error(e);
+        }
+    }
+
+  private:
+    void loop() {
+        for (;;) {
+            OutputJob job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return done_ || !q_.empty(); });
+                if (q_.empty()) return;
+                job = std::move(q_.front());
+                q_.pop_front();
+            }
+            try {
+                fn_(job);
+            } catch (const std::exception& e) {
+                std::unique_lock<std::mutex> lk(mu_);
+                if (error_.empty()) error_ = e.what();
+            }
+        }
+    }
+    Fn fn_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<OutputJob> q_;
+    bool done_ = false;
+    std::string error_;
+    std::thread th_;
+};
+
 struct Run {
     RunSettings st;
     RefMesh mesh;
@@ -127,7 +200,15 @@ struct Run {
     ms_mesh_info_out info{};
     double upper_slide_z = 0.0, lower_slide_z = 0.0;
     ms_ctx* ctx = nullptr;
-    ~Run() { ms_destroy(ctx); }
+    std::unique_ptr<OutputWriter> writer;
+    ~Run() {
+        if (writer) try {
+                writer->finish();
+            } catch (const std::exception& e) {
+                std::cerr << e.what() << std::endl;
+            }
+        ms_destroy(ctx);
+    }
 };
 
 ms_scales scales_for(const Run& r, double damping_prefactor, bool gradient_descent) {
@@ -163,26 +244,35 @@ ms_shell_params params_for(const Run& r, const ms_scales& s, bool gradient_desce
     return p;
 }
 
-// calc_curvatures, calc_angle_deficits, calc_energy_densities, calc_strains
-// and write_output at the current state (main.cpp:409-437).
-void write_output(Run& r, int64_t stepcount, double dial_factor, int status) {
-    const int N = r.mesh.num_nodes, T = r.mesh.num_tris;
-    std::vector<double> dofs(3 * (size_t)N), gauss(T), mean(T), strain(T), dens(2 * (size_t)T), deficit;
-    check(ms_get_state(r.ctx, 0, dofs.data(), nullptr), "ms_get_state");
-    check(ms_calc_diagnostics(r.ctx, 0, gauss.data(), mean.data(), strain.data()), "ms_calc_diagnostics");
-    check(ms_calc_energy_densities(r.ctx, 0, dens.data()), "ms_calc_energy_densities");
-    const double scale = r.sc.char_stretch_energy_density_scale;
-    for (auto& d : dens) d /= scale;
-    angle_deficits(r.mesh, r.boundary, dofs.data(), deficit);
-    OutputFields f{dofs.data(), gauss.data(), mean.data(), dens.data(), dens.data() + T, strain.data(),
+// Formats and writes one output (writer thread): angle deficits, the VTK file
+// and the aux_output.txt row (main.cpp:409-437, write_output.cpp).
+void write_job(const Run& r, const OutputJob& j) {
+    const int T = r.mesh.num_tris;
+    std::vector<double> deficit;
+    angle_deficits(r.mesh, r.boundary, j.dofs.data(), deficit);
+    OutputFields f{j.dofs.data(), j.gauss.data(), j.mean.data(), j.dens.data(), j.dens.data() + T, j.strain.data(),
                    deficit.data()};
-    write_output_vtk(r.out_dir, stepcount, dial_factor, r.mesh, f);
+    write_output_vtk(r.out_dir, j.stepcount, j.dial_factor, r.mesh, f);
     std::ofstream aux(r.aux_path, std::ios::app);
-    if (stepcount == 0)
+    if (j.stepcount == 0)
         aux << "stepcount  dial_factor  stuff.status  upper_slide_z  upper_slide_force  lower_slide_z  "
                "lower_slide_force  \n";
-    aux << stepcount << "  " << dial_factor << " " << status << "  " << r.upper_slide_z << " " << 0.0 << " "
+    aux << j.stepcount << "  " << j.dial_factor << " " << j.status << "  " << r.upper_slide_z << " " << 0.0 << " "
         << r.lower_slide_z << " " << 0.0 << " \n";
+}
+
+// The device part of an output (state, curvatures, strain, energy
+// densities), copied to the host and handed to the writer thread.
+void write_output(Run& r, int64_t stepcount, double dial_factor, int status) {
+    const int N = r.mesh.num_nodes, T = r.mesh.num_tris;
+    OutputJob j{stepcount, dial_factor, status, std::vector<double>(3 * (size_t)N), std::vector<double>(T),
+                std::vector<double>(T), std::vector<double>(T), std::vector<double>(2 * (size_t)T)};
+    check(ms_get_state(r.ctx, 0, j.dofs.data(), nullptr), "ms_get_state");
+    check(ms_calc_diagnostics(r.ctx, 0, j.gauss.data(), j.mean.data(), j.strain.data()), "ms_calc_diagnostics");
+    check(ms_calc_energy_densities(r.ctx, 0, j.dens.data()), "ms_calc_energy_densities");
+    const double scale = r.sc.char_stretch_energy_density_scale;
+    for (auto& d : j.dens) d /= scale;
+    r.writer->submit(std::move(j));
 }
 
 int run(int argc, char** argv) {
@@ -324,6 +414,7 @@ int run(int argc, char** argv) {
     rp.pause_on_eq = 1;
     check(ms_relax_setup(r.ctx, 0, &rp), "ms_relax_setup");
 
+    r.writer = std::make_unique<OutputWriter>([&r](const OutputJob& j) { write_job(r, j); });
     r.log.precision(8, true);
     r.log << "Beginning dynamics loop." << std::endl;
     const auto t0 = std::chrono::steady_clock::now();
@@ -344,6 +435,7 @@ int run(int argc, char** argv) {
         if (rep.failed) {
             // main.cpp:440-470: output of the failing step's input, then the error
             write_output(r, steps, rep.dial_factor, rep.status);
+            r.writer->finish();
             r.log << "At stepcount = " << steps << ", dial_factor = " << rep.dial_factor
                   << " there was a problem: implausibly_high_force" << std::endl;
             return -1;
@@ -375,6 +467,7 @@ int run(int argc, char** argv) {
             break;
         }
     }
+    r.writer->finish();  // every output written
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     double E[3] = {0, 0, 0};
     check(ms_calc_energies(r.ctx, 0, E), "calc_energies");

@@ -382,30 +382,40 @@ def train(net: WarmStartNet, data: list[tuple], epochs: int = 60, graphs_per_ste
 # network's errors cost stretch energy, so a shorter step along the
 # prediction often beats both the flat start and the full prediction
 ALPHAS = (1.0, 0.75, 0.5, 0.35, 0.25, 0.15)
+# the network is trained up to z -> -z (its loss takes the better
+# reflection), but a programmed bbar prefers one side: both are tried
+SIGNS = (1.0, -1.0)
 
 
 def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000) -> tuple[list, list]:
     """Start plans with the energy fallback.  For each shell the predicted
-    displacement is scaled by the alpha of least elastic energy (at dial
-    factor 1); the scaled prediction starts at dial factor 1 when its energy
-    is below that of the flat start with noise, otherwise the flat start is
-    dialled as usual.  Returns (plans, chosen alpha or None)."""
+    displacement, in either z orientation, is scaled by the alpha of least
+    elastic energy (at dial factor 1); the scaled prediction starts at dial
+    factor 1 when its energy is below that of the flat start with noise,
+    otherwise the flat start is dialled as usual.  Returns (plans, the chosen
+    signed alpha or None)."""
     from .workloads import flat_noisy_state
 
     flats = [flat_noisy_state(sh, seed=i)[0] for i, sh in enumerate(shells)]
-    cands = []
+    cands, labels = [], [(s, a) for s in SIGNS for a in ALPHAS]
     for sh in shells:
-        pred = net.predict_dofs(sh)
-        ref = sh.ref_pos
-        cands.append([ref + a * (pred - ref) for a in ALPHAS])
+        d = net.predict_dofs(sh) - sh.ref_pos
+        N = sh.N
+        one = []
+        for s, a in labels:
+            dd = a * d
+            dd[2 * N:] *= s
+            one.append(sh.ref_pos + dd)
+        cands.append(one)
     e_pred = full_dial_energies(shells, cands)
     e_flat = full_dial_energies(shells, flats)
     plans, used = [], []
     for i in range(len(shells)):
-        k = int(np.nanargmin(e_pred[i])) if np.isfinite(e_pred[i]).any() else -1
-        ok = k >= 0 and e_pred[i][k] < e_flat[i]
+        e = np.asarray(e_pred[i])
+        k = int(np.nanargmin(e)) if np.isfinite(e).any() else -1
+        ok = k >= 0 and e[k] < e_flat[i]
         plans.append((cands[i][k], 1.0, 1) if ok else None)
-        used.append(ALPHAS[k] if ok else None)
+        used.append(labels[k][0] * labels[k][1] if ok else None)
     return plans, used
 
 
@@ -460,7 +470,13 @@ def main(argv=None) -> int:
     ap.add_argument("--max-steps", type=int, default=600000)
     ap.add_argument("--data", default="", help="npz of generated equilibria: loaded if present, else written")
     ap.add_argument("--family", default="defect", choices=FAMILIES)
+    ap.add_argument("--params", default="", help="trained parameters to evaluate (skips generation and training)")
     args = ap.parse_args(argv)
+    if args.params:
+        net = WarmStartNet.load(args.params)
+        ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family)
+        print(json.dumps({"params": args.params, "eval": ev}), flush=True)
+        return 0
     t0 = time.time()
     if args.data and os.path.exists(args.data):
         data = load_equilibria(args.data)

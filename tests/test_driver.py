@@ -164,3 +164,29 @@ def test_driver_run_matches_context_relax(tmp_path, spacing):
     assert eq_step == last_step
     # the output prints 15 significant digits
     assert np.allclose(x_drv, x_ctx, rtol=0, atol=1e-13 * np.abs(x_ctx).max() + 1e-300)
+
+
+@pytest.mark.gpu
+def test_driver_regular_outputs(tmp_path):
+    """outputs_per_char_long_time > 0: an output at step 0 and every
+    steps_between_outputs steps (written by the writer thread while the
+    device steps on), plus the equilibrium output."""
+    settings = SETTINGS.format(spacing=1.0).replace("outputs_per_char_long_time = -1.0;",
+                                                    "outputs_per_char_long_time = 20.0;")
+    m, ref, st, ans, _ = _inputs(tmp_path, settings=settings)
+    r = _run(tmp_path, st, ref, ans)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = tmp_path / "output_using_settings_test.txt"
+    steps = sorted(int(p.name.split("_")[1]) for p in out.glob("stepcount_*_output.vtk"))
+    assert steps[0] == 0 and len(steps) >= 3
+    every = steps[1]
+    regular, last = steps[:-1], steps[-1]
+    assert regular == [k * every for k in range(len(regular))]
+    assert last > regular[-1]
+    aux = (out / "aux_output.txt").read_text().strip().split("\n")
+    assert aux[0].startswith("stepcount") and len(aux) == len(steps) + 1
+    assert [int(ln.split()[0]) for ln in aux[1:]] == steps
+    for s in steps:
+        lines, x = _read_output(out / f"stepcount_{s}_output.vtk", m.num_nodes)
+        assert np.isfinite(x).all()
+        assert lines[-1] == "" and "POINT_DATA" in " ".join(lines[-m.num_nodes - 4:])
