@@ -361,7 +361,13 @@ def bench_relax(args) -> dict:
     out = {"metric": "time-to-equilibrium", "unit": "s", "higher_is_better": False,
            "config": {"workload": f"small defect disc, {sh.T} triangles, LCE lambda 1->0.9 over {K} steps, "
                                   f"h=0.01, force tol {tol_f}, speed tol {tol_v}"}}
+    from morphoshell_amd import autotune
+
+    space = dict(autotune.DEFAULT_SPACE, node_block=(64, 128), fused_node_block=(64, 128))
     with Context(sh, precision="fp64") as ctx:
+        ctx.set_state(0, x0, v0)
+        tuned64 = autotune.tune(ctx, steps=400, space=space, use_cache=False)
+        log(f"fp64 tuning: {tuned64}")
         arm(ctx)
         t0 = time.perf_counter()
         r64 = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
@@ -370,10 +376,12 @@ def bench_relax(args) -> dict:
         E64 = ctx.calc_energies(0)
     log(f"fp64: {r64} in {t64:.2f}s")
     with Context(sh, precision="mixed") as ctx:
+        ctx.set_state(0, x0, v0)
+        tunedm = autotune.tune(ctx, steps=400, space=space, use_cache=False)
         arm(ctx, mixed_tol, mixed_tol)
         t0 = time.perf_counter()
         rm = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
-        steps_mixed = ctx.steps_taken
+        steps_mixed = rm["steps"]
         ctx.set_precision("fp64")
         # continue from the mixed end state, already fully dialled
         ctx.relax_setup(0, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=interval,
@@ -382,7 +390,7 @@ def bench_relax(args) -> dict:
         tm = time.perf_counter() - t0
         xm, _ = ctx.get_state(0)
         Em = ctx.calc_energies(0)
-        steps_total = ctx.steps_taken
+        steps_total = rm["steps"] + rf["steps"]
     log(f"mixed: {rm}, then fp64 {rf}, {tm:.2f}s")
     # the CPU oracle runs the same protocol for a bounded time; the rest is
     # extrapolated at its measured per-step rate
@@ -405,6 +413,7 @@ def bench_relax(args) -> dict:
                             abs(E64[0] + E64[1]),
                             "max_position_diff_vs_fp64": float(np.abs(xm - x64).max())},
         "speedup_mixed_vs_fp64": t64 / tm if tm > 0 else None,
+        "tuning": {"fp64": tuned64, "mixed": tunedm},
         "cpu_baseline": {"seconds_extrapolated": cpu_per_step * n64, "per_step_s": cpu_per_step,
                          "cores": orc.lib().orc_num_threads(), "kind": "port",
                          "sample": f"{steps_cpu} steps of the same protocol, extrapolated to {n64} steps"},

@@ -308,7 +308,13 @@ int run(int argc, char** argv) {
     double dial_start = 0.0;
     std::vector<double> ansatz;
     try {
-        r.mesh = read_ref_vtk(ref_path, r.st.ref_thickness_if_uniform, r.st.ref_shear_modulus_if_uniform);
+        bool cached = false;
+        const auto tr = std::chrono::steady_clock::now();
+        r.mesh = read_ref_vtk_cached(ref_path, r.st.ref_thickness_if_uniform, r.st.ref_shear_modulus_if_uniform,
+                                     cached);
+        r.log << (cached ? "Read the reference data from its binary sidecar " : "Read the reference data file ")
+              << "in " << std::chrono::duration<double>(std::chrono::steady_clock::now() - tr).count() << " s"
+              << std::endl;
         if (!ansatz_path.empty()) read_ansatz_vtk(ansatz_path, r.mesh.num_nodes, ansatz, dial_start);
     } catch (const std::exception& e) {
         r.log << e.what() << "\nAn error while reading data likely implies a problem with a data file, for example "

@@ -4,7 +4,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <chrono>
 #include <iomanip>
 #include <limits>
 #include <stdexcept>
@@ -146,6 +149,115 @@ RefMesh read_ref_vtk(const std::string& path, double uniform_thickness, double u
     for (int t = 0; t < 3 * T; ++t)
         if (m.tris[t] < 0 || m.tris[t] >= N)
             throw std::runtime_error("Error: a triangle refers to a node id outside [0, number of nodes).");
+    return m;
+}
+
+namespace {
+
+constexpr char kMagic[8] = {'M', 'S', 'B', 'I', 'N', '0', '0', '1'};
+
+struct SidecarKey {
+    uint64_t size;
+    int64_t mtime;
+};
+
+SidecarKey key_of(const std::string& path) {
+    namespace fs = std::filesystem;
+    const auto t = fs::last_write_time(path).time_since_epoch();
+    return {(uint64_t)fs::file_size(path), (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t).count()};
+}
+
+template <class T>
+void put(std::ofstream& o, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    o.write(reinterpret_cast<const char*>(&n), sizeof n);
+    o.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+}
+template <class T>
+bool get(std::ifstream& in, std::vector<T>& v, uint64_t expect) {
+    uint64_t n = 0;
+    in.read(reinterpret_cast<char*>(&n), sizeof n);
+    if (!in || n != expect) return false;
+    v.resize(n);
+    in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+    return (bool)in;
+}
+
+bool load_sidecar(const std::string& bin, const SidecarKey& k, RefMesh& m) {
+    std::ifstream in(bin, std::ios::binary);
+    if (!in) return false;
+    char magic[8];
+    SidecarKey kk{};
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char*>(&kk), sizeof kk);
+    if (!in || std::memcmp(magic, kMagic, 8) != 0 || kk.size != k.size || kk.mtime != k.mtime) return false;
+    int32_t hdr[4];
+    in.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    if (!in || hdr[0] <= 0 || hdr[1] <= 0 || hdr[2] <= 0 || hdr[3] <= 0) return false;
+    m.num_nodes = hdr[0];
+    m.num_tris = hdr[1];
+    m.abar_cols = hdr[2];
+    m.bbar_cols = hdr[3];
+    const uint64_t N = (uint64_t)m.num_nodes, T = (uint64_t)m.num_tris;
+    if (!(get(in, m.pos, 3 * N) && get(in, m.tris, 3 * T) && get(in, m.abar_info, T * m.abar_cols) &&
+          get(in, m.bbar_info, T * m.bbar_cols) && get(in, m.thickness, T) && get(in, m.shear_modulus, T) &&
+          get(in, m.tri_tags, T) && get(in, m.constrained, N) && get(in, m.node_tags, N)))
+        return false;
+    for (int32_t v : m.tris)
+        if (v < 0 || v >= m.num_nodes) return false;
+    for (uint8_t c : m.constrained)
+        if (c > 1) return false;
+    return true;
+}
+
+void save_sidecar(const std::string& bin, const SidecarKey& k, const RefMesh& m) {
+    const std::string tmp = bin + ".tmp";
+    {
+        std::ofstream o(tmp, std::ios::binary);
+        if (!o) return;
+        o.write(kMagic, 8);
+        o.write(reinterpret_cast<const char*>(&k), sizeof k);
+        const int32_t hdr[4] = {m.num_nodes, m.num_tris, m.abar_cols, m.bbar_cols};
+        o.write(reinterpret_cast<const char*>(hdr), sizeof hdr);
+        put(o, m.pos);
+        put(o, m.tris);
+        put(o, m.abar_info);
+        put(o, m.bbar_info);
+        put(o, m.thickness);
+        put(o, m.shear_modulus);
+        put(o, m.tri_tags);
+        put(o, m.constrained);
+        put(o, m.node_tags);
+        if (!o) return;
+    }
+    std::error_code ec;
+    std::filesystem::rename(tmp, bin, ec);
+}
+
+}  // namespace
+
+RefMesh read_ref_vtk_cached(const std::string& path, double uniform_thickness, double uniform_shear_modulus,
+                            bool& from_sidecar) {
+    from_sidecar = false;
+    std::error_code ec;
+    if (!std::filesystem::exists(path, ec)) throw std::runtime_error("Error: Problem opening reference data file.");
+    const SidecarKey k = key_of(path);
+    const std::string bin = path + ".msbin";
+    RefMesh m;
+    if (load_sidecar(bin, k, m)) {
+        from_sidecar = true;
+        for (auto& h : m.thickness)
+            if (uniform_thickness > 0) h = uniform_thickness;
+        for (auto& mu : m.shear_modulus)
+            if (uniform_shear_modulus > 0) mu = uniform_shear_modulus;
+        return m;
+    }
+    m = read_ref_vtk(path, -1.0, -1.0);  // the file's own values go into the sidecar
+    save_sidecar(bin, k, m);
+    for (auto& h : m.thickness)
+        if (uniform_thickness > 0) h = uniform_thickness;
+    for (auto& mu : m.shear_modulus)
+        if (uniform_shear_modulus > 0) mu = uniform_shear_modulus;
     return m;
 }
 

@@ -33,6 +33,13 @@ struct RefMesh {
 // thickness / shear modulus settings (> 0) replace the file's values.
 RefMesh read_ref_vtk(const std::string& path, double uniform_thickness, double uniform_shear_modulus);
 
+// The same through a binary sidecar <path>.msbin: loaded instead of parsing
+// the text when its recorded size and modification time match the text
+// file's, otherwise written after a parse (a failed write is not an error).
+// from_sidecar tells which happened.
+RefMesh read_ref_vtk_cached(const std::string& path, double uniform_thickness, double uniform_shear_modulus,
+                            bool& from_sidecar);
+
 // Ansatz positions (3N dofs SoA) and the dial factor to start from.
 void read_ansatz_vtk(const std::string& path, int32_t num_nodes, std::vector<double>& pos, double& dial_factor);
 

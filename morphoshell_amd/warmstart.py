@@ -278,6 +278,7 @@ def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int 
 
     t0 = time.perf_counter()
     with Context(shells, precision=precision) as ctx:
+        ctx.set_tuning(node_block=128, fused=1)  # small shells: more, smaller tiles
         for i, sh in enumerate(shells):
             plan = None if plans is None else plans[i]
             if plan is None:

@@ -190,3 +190,24 @@ def test_driver_regular_outputs(tmp_path):
         lines, x = _read_output(out / f"stepcount_{s}_output.vtk", m.num_nodes)
         assert np.isfinite(x).all()
         assert lines[-1] == "" and "POINT_DATA" in " ".join(lines[-m.num_nodes - 4:])
+
+
+def test_driver_binary_sidecar(tmp_path):
+    """The first run writes <mesh>.msbin; the next one reads it; a changed
+    text file is parsed again."""
+    import time
+
+    m, ref, st, ans, _ = _inputs(tmp_path)
+    logs = []
+    for k in range(3):
+        if k == 2:
+            time.sleep(0.01)
+            ref.write_text(ref.read_text())  # same content, new modification time
+        _run(tmp_path, st, ref, ans)
+        logs.append((tmp_path / "output_using_settings_test.txt" / "log.txt").read_text())
+    assert (tmp_path / "disc.vtk.msbin").exists()
+    assert "Read the reference data file in" in logs[0]
+    assert "Read the reference data from its binary sidecar" in logs[1]
+    assert "Read the reference data file in" in logs[2]
+    for log in logs:
+        assert f"Number of nodes = {m.num_nodes}" in log and f"Number of triangles = {m.num_tris}" in log
