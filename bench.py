@@ -380,7 +380,16 @@ def bench_relax(args) -> dict:
         tunedm = autotune.tune(ctx, steps=400, space=space, use_cache=False)
         arm(ctx, mixed_tol, mixed_tol)
         t0 = time.perf_counter()
-        rm = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
+        # the mixed phase ends at its tolerance or when its residual stops
+        # falling (FP32 force noise floors it), whichever comes first
+        chunk, best = 100_000, float("inf")
+        while True:
+            rm = ctx.relax_run(chunk, poll_every=20000)[0]
+            if rm["status"] == 2 or rm["failed"] or rm["steps"] >= args.relax_steps:
+                break
+            if rm["status"] == 1 and rm["max_force_ratio"] > 0.9 * best:
+                break
+            best = min(best, rm["max_force_ratio"]) if rm["status"] == 1 else best
         steps_mixed = rm["steps"]
         ctx.set_precision("fp64")
         # continue from the mixed end state, already fully dialled
