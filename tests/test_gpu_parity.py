@@ -450,3 +450,62 @@ def test_output_diagnostics_match_reference_forms():
         err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
         print(f"{name}: rel err {err:.2e}")
         assert err <= 1e-10, name
+
+
+def test_create_and_call_argument_errors(sh):
+    """Bad inputs are refused with MS_ERR_ARG and a message, nothing runs."""
+    import copy
+
+    from morphoshell_amd.native import MS_ERR_ARG, NativeError
+
+    bad = copy.copy(sh)
+    bad.patch_ids = sh.patch_ids.copy()
+    bad.patch_ids[5, 4] = sh.N + 3
+    with pytest.raises(NativeError, match="patch id out of range") as ei:
+        _ctx(bad)
+    assert ei.value.code == MS_ERR_ARG
+    bad = copy.copy(sh)
+    bad.mesh = copy.copy(sh.mesh)
+    bad.mesh.abar_info = sh.mesh.abar_info[:, :2].copy()
+    with pytest.raises(NativeError, match="3-column"):
+        _ctx(bad)
+    with _ctx(sh) as ctx:
+        with pytest.raises(NativeError, match="out of range"):
+            ctx.set_state(1, sh.ref_pos)
+        with pytest.raises(NativeError, match="fused must be"):
+            ctx.set_tuning(fused=3)
+        with pytest.raises(NativeError, match="node_block"):
+            ctx.set_tuning(node_block=96)
+        with pytest.raises(NativeError, match="nsteps"):
+            ctx.step(-1)
+        # the context still works after refused calls
+        ctx.set_state(0, sh.ref_pos)
+        ctx.step(2)
+
+
+def test_autotuner_picks_a_shape_without_changing_results(tmp_path, monkeypatch):
+    """autotune.tune times the candidates, applies the fastest, caches it by
+    workload signature; results are the same bits under any choice."""
+    from morphoshell_amd import autotune
+
+    monkeypatch.setenv("MS_TUNE_CACHE", str(tmp_path / "tune.json"))
+    shells = [_shell(0.08, seed=1), _shell(0.05, seed=2, kappa=-1.0)]
+    states = [deformed_state(s, seed=70 + i) for i, s in enumerate(shells)]
+    space = {"tri_block": (64, 256), "node_block": (128,), "grid_per_cu": (0, 4), "graph_steps": (8,),
+             "fused_node_block": (64, 256)}
+    with _ctx(shells) as a, _ctx(shells) as b:
+        best = autotune.tune(a, steps=10, space=space)
+        assert not best["cached"] and best["candidates"] == 8
+        t = a.get_tuning()
+        assert t["fused"] == best["fused"] and t["node_block"] == best["node_block"]
+        again = autotune.tune(b, steps=10, space=space)
+        assert again["cached"] and again["fused"] == best["fused"]
+        for c in (a, b):
+            for i, st in enumerate(states):
+                c.set_state(i, *st)
+                c.resume(i)
+        a.step(20)
+        b.set_tuning(tri_block=256, node_block=256, tri_grid_per_cu=0, node_grid_per_cu=0, fused=0)
+        b.step(20)
+        for i in range(2):
+            assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0])

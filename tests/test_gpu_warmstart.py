@@ -96,3 +96,31 @@ def test_training_is_deterministic_and_fits():
     assert np.array_equal(params[0], params[1])
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
     del rng, N
+
+
+def test_trained_warm_start_lowers_the_start_energy():
+    """End to end on small defect shells: equilibria from the batched solver,
+    training on the device, then on unseen shells the chosen starts have a
+    lower elastic energy (at dial factor 1) than the flat start with noise,
+    and relaxations from them reach equilibrium (the plain dynamics finish)."""
+    from morphoshell_amd import warmstart as ws
+    from morphoshell_amd.workloads import flat_noisy_state
+
+    data = ws.generate(24, seed=3, batch=24, tol=1e-4, max_steps=300_000, family="defect")
+    assert len(data) >= 12
+    net = ws.WarmStartNet(seed=0, width=32, layers=3)
+    hist = ws.train(net, data, epochs=60, graphs_per_step=4, lr=3e-3)
+    assert hist[-1] < 0.6 * hist[0], hist
+    shells = ws.training_family(8, 20_000, family="defect")
+    plans, used = ws.warm_start_plans(net, shells)
+    flats = [flat_noisy_state(sh, seed=i)[0] for i, sh in enumerate(shells)]
+    e_flat = ws.full_dial_energies(shells, flats)
+    e_warm = ws.full_dial_energies(shells, [p[0] if p else f for p, f in zip(plans, flats)])
+    n_used = sum(u is not None for u in used)
+    print(f"predictions used {n_used}/8, start energy ratio median "
+          f"{np.median(np.array(e_warm) / np.array(e_flat)):.3f}")
+    assert n_used >= 6
+    for p, ew, ef in zip(plans, e_warm, e_flat):
+        assert (ew < ef) if p else (ew == ef)
+    reps, _, _ = ws.relax_batch(shells, 1e-4, 400_000, plans=plans)
+    assert sum(r["status"] == 2 for r in reps) >= 6
