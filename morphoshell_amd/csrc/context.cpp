@@ -54,16 +54,13 @@ struct ms_ctx {
     size_t part_len = 0;
 
     // fused tile step (built on demand for the current node chunks)
-    int* tile_off = nullptr;
-    int* tile_tris = nullptr;
-    unsigned short* tile_dst = nullptr;
-    int tile_max_entries = 0;
-    int tile_T = 0;                // total tile entries (triangles counted per tile)
+    TileData tile{};                 // device metadata + tile-ordered records
+    int tile_T = 0;                  // total tile entries (triangles counted per tile)
     bool tiles_ready = false;
+    std::vector<void*> tile_meta;    // metadata allocations
     std::vector<void*> tile_allocs;  // tile-ordered record copies
     int64_t tile_bytes = 0;          // metadata
     int64_t tile_rec_bytes = 0;      // record copies
-    TriData tile_td{};
     int acols = 3;
 
     // graphs
@@ -143,30 +140,28 @@ int build_chunks(ms_ctx* c) {
 void free_tile_records(ms_ctx* c) {
     for (void* p : c->tile_allocs) hipFree(p);
     c->tile_allocs.clear();
-    c->tile_td = TriData{};
+    c->tile.td = TriData{};
     c->dev_bytes -= c->tile_rec_bytes;
     c->tile_rec_bytes = 0;
 }
 
 void free_tiles(ms_ctx* c) {
     free_tile_records(c);
-    if (c->tile_off) hipFree(c->tile_off);
-    if (c->tile_tris) hipFree(c->tile_tris);
-    if (c->tile_dst) hipFree(c->tile_dst);
-    c->tile_off = c->tile_tris = nullptr;
-    c->tile_dst = nullptr;
-    c->tile_max_entries = 0;
+    for (void* p : c->tile_meta) hipFree(p);
+    c->tile_meta.clear();
+    c->tile = TileData{};
     c->tile_T = 0;
     c->dev_bytes -= c->tile_bytes;
     c->tile_bytes = 0;
     c->tiles_ready = false;
 }
 
-// Tile-ordered copies of every per-triangle stream (gathered on the device).
+// Tile-ordered copies of the per-triangle streams the mechanics reads
+// (gathered on the device; the patch ids are replaced by tile-local ids).
 int build_tile_records(ms_ctx* c) {
     free_tile_records(c);
     const int T = c->T, Tp = c->tile_T;
-    TriData& o = c->tile_td;
+    TriData& o = c->tile.td;
     o.T = Tp;
     o.N = c->N;
     auto copy = [&](const void* in, int elem_bytes, int ncomp, const void** out) -> hipError_t {
@@ -179,11 +174,10 @@ int build_tile_records(ms_ctx* c) {
         c->tile_allocs.push_back(p);
         c->tile_rec_bytes += (int64_t)nb;
         c->dev_bytes += (int64_t)nb;
-        launch_tile_gather(c->stream, in, p, elem_bytes, ncomp, T, Tp, c->tile_tris);
+        launch_tile_gather(c->stream, in, p, elem_bytes, ncomp, T, Tp, c->tile.tris);
         *out = p;
         return hipGetLastError();
     };
-    HIPC(c, copy(c->pid, 4, 6, (const void**)&o.pid));
     HIPC(c, copy(c->G, 8, 6, (const void**)&o.G));
     HIPC(c, copy(c->H, 8, 18, (const void**)&o.H));
     HIPC(c, copy(c->G32, 4, 6, (const void**)&o.G32));
@@ -200,12 +194,15 @@ int build_tile_records(ms_ctx* c) {
     return MS_OK;
 }
 
-// Tile metadata of the fused step for the current node chunks: per tile the
-// ascending list of triangles whose patch touches one of its nodes, and for
-// each (triangle, slot) the LDS position of its patch-incidence entry.
+// Tile metadata of the fused step for the current node chunks (see TileData):
+// per tile the ascending triangles whose patch touches one of its nodes, the
+// halo nodes they reach, tile-local patch ids, and per pass the LDS positions
+// of the patch-incidence entries and each node's first position.
 int build_tiles(ms_ctx* c) {
     free_tiles(c);
     const int T = c->T, N = c->N;
+    const int B = c->tune.node_block;
+    const int split = 2 * B;
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
     std::vector<Chunk> nc(c->n_node_chunks);
@@ -214,56 +211,113 @@ int build_tiles(ms_ctx* c) {
     ent.resize((size_t)off[N]);
     HIPC(c, hipMemcpy(ent.data(), c->csr_ent, ent.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPC(c, hipMemcpy(nc.data(), c->d_node_chunks, nc.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
-    std::vector<int> toff(nc.size() + 1, 0), tris;
-    std::vector<unsigned short> dst;
-    int max_entries = 0;
+    const int stride = B + 1;
+    std::vector<int> toff(nc.size() + 1, 0), tris, hoff(nc.size() + 1, 0), halo;
+    std::vector<unsigned short> lpid, dst, poff(nc.size() * 2 * (size_t)stride, 0);
+    int max_entries = 0, max_nodes = 0;
+    std::vector<int> tl, hl, cnt0;
     for (size_t ci = 0; ci < nc.size(); ++ci) {
-        const int n0 = nc[ci].x, n1 = nc[ci].y;
-        const long long base = off[n0];
-        const long long cnt = off[n1] - base;
-        if (cnt >= 0xFFFF)
-            return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu has %lld patch entries (limit 65534)", ci, cnt));
-        max_entries = std::max(max_entries, (int)cnt);
-        std::vector<int> tl;
-        tl.reserve((size_t)cnt);
-        for (long long e = base; e < off[n1]; ++e) tl.push_back(ent[e] / 6);
+        const int n0 = nc[ci].x, n1 = nc[ci].y, nn = n1 - n0;
+        if (nn > B) return fail(c, MS_ERR_ARG, "fused step: a node chunk is larger than the block");
+        tl.clear();
+        for (long long e = off[n0]; e < off[n1]; ++e) tl.push_back(ent[e] / 6);
         std::sort(tl.begin(), tl.end());
         tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
-        for (int t : tl) {
+        hl.clear();
+        for (int t : tl)
+            for (int s = 0; s < 6; ++s) {
+                const int p = pid[(size_t)s * T + t];
+                if (p < n0 || p >= n1) hl.push_back(p);
+            }
+        std::sort(hl.begin(), hl.end());
+        hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+        if (nn + (long long)hl.size() > 0xFFFF)
+            return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu reaches too many nodes", ci));
+        max_nodes = std::max(max_nodes, nn + (int)hl.size());
+        // entries of each own node in the first pass: those of its triangles
+        // among the tile's first `split` (a prefix of the node's CSR range)
+        const long long t_split = (long long)tl.size() > split ? tl[split] : (long long)INT_MAX + 1;
+        cnt0.assign(nn, 0);
+        unsigned short* p0 = &poff[(2 * ci) * (size_t)stride];
+        unsigned short* p1 = p0 + stride;
+        for (int i = 0; i < nn; ++i) {
+            const long long b = off[n0 + i], e = off[n0 + i + 1];
+            long long k = b;
+            while (k < e && ent[k] / 6 < t_split) ++k;
+            cnt0[i] = (int)(k - b);
+            const long long a0 = p0[i] + (long long)cnt0[i], a1 = p1[i] + (e - b - cnt0[i]);
+            if (a0 > 0xFFFE || a1 > 0xFFFE)
+                return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu has too many patch entries", ci));
+            p0[i + 1] = (unsigned short)a0;
+            p1[i + 1] = (unsigned short)a1;
+        }
+        max_entries = std::max(max_entries, std::max((int)p0[nn], (int)p1[nn]));
+        for (size_t jj = 0; jj < tl.size(); ++jj) {
+            const int t = tl[jj];
+            const int pass = (int)jj < split ? 0 : 1;
             tris.push_back(t);
             for (int s = 0; s < 6; ++s) {
                 const int p = pid[(size_t)s * T + t];
-                unsigned short q = 0xFFFF;
+                unsigned short q = 0xFFFF, l;
                 if (p >= n0 && p < n1) {
+                    const int i = p - n0;
+                    l = (unsigned short)i;
                     const int key = t * 6 + s;
                     const int* b = ent.data() + off[p];
                     const int* e = ent.data() + off[p + 1];
                     const int* it = std::lower_bound(b, e, key);
                     if (it == e || *it != key)
                         return fail(c, MS_ERR_ARG, "fused step: patch-incidence table is inconsistent");
-                    q = (unsigned short)((it - ent.data()) - base);
+                    const int rank = (int)(it - b) - (pass ? cnt0[i] : 0);
+                    q = (unsigned short)((pass ? p1[i] : p0[i]) + rank);
+                } else {
+                    l = (unsigned short)(nn + (std::lower_bound(hl.begin(), hl.end(), p) - hl.begin()));
                 }
+                lpid.push_back(l);
                 dst.push_back(q);
             }
         }
         toff[ci + 1] = (int)tris.size();
+        halo.insert(halo.end(), hl.begin(), hl.end());
+        hoff[ci + 1] = (int)halo.size();
     }
-    const int lds64 = tile_lds_bytes(max_entries, false), lds32 = tile_lds_bytes(max_entries, true);
-    if (lds64 > 160 * 1024)
+    TileData& td = c->tile;
+    td.poff_stride = stride;
+    td.pass_split = split;
+    td.max_nodes = max_nodes;
+    td.max_entries = max_entries;
+    const int lds64 = tile_lds_bytes(td, false), lds32 = tile_lds_bytes(td, true);
+    if (lds64 + 1024 > 160 * 1024)
         return fail(c, MS_ERR_ARG, fmt("fused step: a tile needs %d bytes of LDS (limit 160 KiB)", lds64));
     HIPC(c, prepare_tile_kernels(lds64, lds32));
-    HIPC(c, hipMalloc((void**)&c->tile_off, toff.size() * sizeof(int)));
-    HIPC(c, hipMalloc((void**)&c->tile_tris, std::max<size_t>(1, tris.size()) * sizeof(int)));
-    HIPC(c, hipMalloc((void**)&c->tile_dst, std::max<size_t>(1, dst.size()) * sizeof(unsigned short)));
-    c->tile_bytes = (int64_t)(toff.size() * sizeof(int) + std::max<size_t>(1, tris.size()) * sizeof(int) +
-                              std::max<size_t>(1, dst.size()) * sizeof(unsigned short));
-    c->dev_bytes += c->tile_bytes;
-    HIPC(c, hipMemcpy(c->tile_off, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!tris.empty()) {
-        HIPC(c, hipMemcpy(c->tile_tris, tris.data(), tris.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIPC(c, hipMemcpy(c->tile_dst, dst.data(), dst.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
-    }
-    c->tile_max_entries = max_entries;
+    auto upload = [&](const auto& v, auto** dptr) -> hipError_t {
+        using E = typename std::decay<decltype(v)>::type::value_type;
+        void* p = nullptr;
+        const size_t nb = std::max<size_t>(1, v.size()) * sizeof(E);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) return e;
+        c->tile_meta.push_back(p);
+        c->tile_bytes += (int64_t)nb;
+        c->dev_bytes += (int64_t)nb;
+        *dptr = static_cast<typename std::remove_pointer<decltype(dptr)>::type>(p);
+        return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice);
+    };
+    int *d_toff, *d_tris, *d_hoff, *d_halo;
+    unsigned short *d_lpid, *d_dst, *d_poff;
+    HIPC(c, upload(toff, &d_toff));
+    HIPC(c, upload(tris, &d_tris));
+    HIPC(c, upload(hoff, &d_hoff));
+    HIPC(c, upload(halo, &d_halo));
+    HIPC(c, upload(lpid, &d_lpid));
+    HIPC(c, upload(dst, &d_dst));
+    HIPC(c, upload(poff, &d_poff));
+    td.tri_off = d_toff;
+    td.tris = d_tris;
+    td.halo_off = d_hoff;
+    td.halo = d_halo;
+    td.lpid = d_lpid;
+    td.dst = d_dst;
+    td.poff = d_poff;
     c->tile_T = (int)tris.size();
     if (int r = build_tile_records(c)) return r;
     c->tiles_ready = true;
@@ -339,9 +393,7 @@ void params_to_dev(const ms_shell_params& p, ShellDev& S) {
     S.from_ansatz = p.dial_from_ansatz ? 1 : 0;
 }
 
-TileData tile_data(const ms_ctx* c) {
-    return TileData{c->tile_off, c->tile_tris, c->tile_dst, c->tile_max_entries, c->tile_td};
-}
+TileData tile_data(const ms_ctx* c) { return c->tile; }
 
 // One step; with the fused tuning the tile launch replaces the triangle and
 // node launches.  timed: events around the two parts (ev[0..2]).

@@ -103,25 +103,36 @@ void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* s
 // Plain 16-byte-vector copy of nbytes (traffic-counter calibration).
 void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
 
-// Fused owner-computes step.  Tile c = node chunk c.  Entries j in
-// [tri_off[c], tri_off[c+1]) are the triangles (ascending) whose patches touch
-// the tile's nodes: tris[j] is the triangle and td holds a copy of its
-// per-triangle record at index j (halo triangles appear in several tiles), so
-// every tile streams its records contiguously.  dst[6j + slot] is the
-// position, relative to csr_off[first node of the tile], of the
-// patch-incidence entry (tris[j], slot), or 0xFFFF when that patch node lies
-// outside the tile.
+// Fused owner-computes step.  Tile c = node chunk c (at most B = block size
+// nodes).  Entries j in [tri_off[c], tri_off[c+1]) are the triangles
+// (ascending) whose patches touch the tile's nodes; td holds a copy of their
+// per-triangle records at index j (halo triangles appear in several tiles),
+// so every tile streams its records contiguously.  The tile's nodes and the
+// halo nodes [halo_off[c], halo_off[c+1]) of its triangles are staged in LDS
+// (tile-local ids: own node n -> n - first, halo node k -> own count + k);
+// lpid[6j + slot] is the tile-local id of a patch node.  The triangles run
+// in two passes (the first pass_split entries, then the rest); dst[6j + slot]
+// is the pass-local LDS position of the patch-incidence entry (j, slot), or
+// 0xFFFF when that node lies outside the tile, and poff[(2c + pass) *
+// poff_stride + i] is the first pass-local position of own node i's entries.
 struct TileData {
     const int* tri_off;            // nchunks + 1
-    const int* tris;
-    const unsigned short* dst;
-    int max_entries;               // max over tiles of the tile's CSR entries
+    const int* tris;               // triangle of entry j
+    const unsigned short* lpid;    // 6 per entry
+    const unsigned short* dst;     // 6 per entry
+    const int* halo_off;           // nchunks + 1
+    const int* halo;               // global ids of halo nodes
+    const unsigned short* poff;    // 2 * poff_stride per tile
+    int poff_stride;               // B + 1
+    int pass_split;                // entries in the first pass (2 B)
+    int max_nodes;                 // max over tiles of own + halo nodes
+    int max_entries;               // max over tiles and passes of pass entries
     TriData td;                    // tile-ordered records, td.T = tri_off[nchunks]
 };
 // out[k*Tp + j] = in[k*T + tris[j]] for k < ncomp, j < Tp (elem_bytes 4 or 8).
 void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp,
                         const int* tris);
-int tile_lds_bytes(int max_entries, bool mixed);
+int tile_lds_bytes(const TileData& tl, bool mixed);
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.
 hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // One fused step of every shell not halted (the work of launch_tri_forces +

@@ -36,13 +36,14 @@ __device__ __forceinline__ double norm3(double a, double b, double c) {
     return sqrt(a * a + b * b + c * c);
 }
 
-// Sum of n contributions (stride 3 per entry, as (x, y, z) triples at
-// base[idx(e)*3]) in the order given; four loads in flight at a time.
+// Adds contributions e0..e1-1 (stride 3 per entry, as (x, y, z) triples at
+// base[idx(e)*3]) to the running sums, one at a time in the order given
+// (four loads in flight), so a sum split over several calls rounds exactly
+// as one call over the whole range.
 template <typename CT, typename IDX>
-__device__ __forceinline__ void ordered_sum3(const CT* base, IDX idx, long long e0, long long e1, double& fx,
-                                             double& fy, double& fz) {
+__device__ __forceinline__ void ordered_sum3_acc(const CT* base, IDX idx, long long e0, long long e1, double& ax,
+                                                 double& ay, double& az) {
 #pragma clang fp contract(off)
-    double ax = 0.0, ay = 0.0, az = 0.0;
     long long e = e0;
     for (; e + 4 <= e1; e += 4) {
         const CT* c0 = base + (size_t)idx(e) * 3;
@@ -70,6 +71,14 @@ __device__ __forceinline__ void ordered_sum3(const CT* base, IDX idx, long long 
         ay = ay + (double)c[1];
         az = az + (double)c[2];
     }
+}
+
+// The sum of contributions e0..e1-1 in the order given.
+template <typename CT, typename IDX>
+__device__ __forceinline__ void ordered_sum3(const CT* base, IDX idx, long long e0, long long e1, double& fx,
+                                             double& fy, double& fz) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    ordered_sum3_acc(base, idx, e0, e1, ax, ay, az);
     fx = ax;
     fy = ay;
     fz = az;

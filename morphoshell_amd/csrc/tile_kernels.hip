@@ -1,14 +1,19 @@
 // Fused owner-computes step (gfx950): one workgroup per node tile.
 //
-// A tile is a node chunk of one shell.  Its workgroup evaluates every
-// triangle whose six-node patch touches one of its nodes (the tile's own
-// triangles plus a halo of about 1.3-1.5x as many, recomputed by each
-// neighbouring tile), writes the contributions its nodes need into LDS at
-// positions fixed by the patch-incidence CSR, and then, one thread per node,
-// sums them in CSR order -- the order of the two-launch path, so the forces
-// are bit-identical to it -- and applies damping, clamp, the equilibrium
-// maxima, the blow-up flag and the update.  The per-slot contributions never
-// go to HBM.
+// A tile is a node chunk of one shell.  Its workgroup
+//   1. stages the positions of its nodes and of the halo nodes its triangles
+//      reach into LDS (coalesced loads, once per tile);
+//   2. evaluates every triangle whose six-node patch touches one of its nodes
+//      (its own triangles plus a halo of about 1.3x as many, recomputed by
+//      each neighbouring tile), reading the patch nodes from LDS through
+//      16-bit tile-local ids, and writes the contributions its nodes need into
+//      LDS in patch-incidence CSR order;
+//   3. one thread per node sums them in that order -- the order of the
+//      two-launch path, so forces are bit-identical to it -- and applies
+//      damping, clamp, the equilibrium maxima, the blow-up flag and the update.
+// Steps 2-3 run in two passes over the tile's triangles (the first 2B, then
+// the rest) with each node's running sum carried over, which halves the LDS
+// the contributions need.  The per-slot contributions never go to HBM.
 
 #include <limits.h>
 
@@ -71,12 +76,20 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
     __syncthreads();
 }
 
+// LDS layout: positions (3 doubles per tile-local node), then contributions
+// (3 CT per pass entry).
+// The mixed kernel is asked for three waves per SIMD (its LDS tile allows
+// them, at the cost of a few spilled registers); the FP64 one is held at two
+// by its LDS tile anyway.
 template <bool MIXED, bool CHECK>
-__global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void tile_step_kernel(
+    TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
-    extern __shared__ unsigned char smem_raw[];
-    CT* lds = reinterpret_cast<CT*>(smem_raw);
+    extern __shared__ double smem[];
+    double* lpos = smem;
+    CT* lcon = reinterpret_cast<CT*>(smem + 3 * (size_t)k.tl.max_nodes);
     const int N = k.nd.N;
+    const int B = blockDim.x;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
         const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
@@ -86,43 +99,67 @@ __global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
         const int par = uni(sh.par);
         const double* X = par ? k.b.x[1] : k.b.x[0];
         const TriParams tp = tri_params(sh);
-        // phase A: contributions of the tile's triangles into LDS, from the
-        // tile's contiguous copy of their records
-        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_off[ci + 1]);
-        for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
-            // opaque to loop strength reduction: the record loads keep one
-            // 32-bit lane offset against scalar stream bases instead of a
-            // 64-bit induction pointer per stream
-            unsigned t = (unsigned)j;
-            asm volatile("" : "+v"(t));
-            unsigned pc[6];
-            load_patch_ids(k.tl.td, t, pc);
-            CT out[18];
-            if (MIXED)
-                tri_forces_mixed(k.tl.td, tp, t, pc, X, reinterpret_cast<float*>(out));
-            else
-                tri_forces_fp64(k.tl.td, tp, t, pc, X, reinterpret_cast<double*>(out));
-            const unsigned short* d = k.tl.dst + (size_t)j * 6;
-#pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                const unsigned int q = d[s];
-                if (q != 0xFFFFu) {
-                    lds[3 * q] = out[3 * s];
-                    lds[3 * q + 1] = out[3 * s + 1];
-                    lds[3 * q + 2] = out[3 * s + 2];
-                }
-            }
+        const int nn = ch.y - ch.x;
+        const int h0 = uni(k.tl.halo_off[ci]), h1 = uni(k.tl.halo_off[ci + 1]);
+        // 1. positions of the tile's nodes and its halo nodes
+        for (int i = threadIdx.x; i < nn + (h1 - h0); i += B) {
+            const unsigned g = i < nn ? (unsigned)(ch.x + i) : (unsigned)k.tl.halo[h0 + i - nn];
+            lpos[3 * i] = ld(X, g);
+            lpos[3 * i + 1] = ld(X + N, g);
+            lpos[3 * i + 2] = ld(X + 2 * (size_t)N, g);
         }
         __syncthreads();
-        // phase B: one thread per node
+        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_off[ci + 1]);
+        const int jm = min(j1, j0 + k.tl.pass_split);
+        const unsigned short* poff = k.tl.poff + (size_t)(2 * ci) * k.tl.poff_stride;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {
+            // 2. this pass's triangles into LDS
+            const int ja = pass ? jm : j0, jb = pass ? j1 : jm;
+            for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
+                // opaque to loop strength reduction: the record loads keep one
+                // 32-bit lane offset against scalar stream bases instead of a
+                // 64-bit induction pointer per stream
+                unsigned t = (unsigned)j;
+                asm volatile("" : "+v"(t));
+                // six 16-bit ids per entry, read as three 32-bit words
+                const unsigned* lw = reinterpret_cast<const unsigned*>(k.tl.lpid);
+                const unsigned w0 = ld(lw, 3 * t), w1 = ld(lw, 3 * t + 1), w2 = ld(lw, 3 * t + 2);
+                const unsigned lp[6] = {w0 & 0xFFFFu, w0 >> 16, w1 & 0xFFFFu, w1 >> 16, w2 & 0xFFFFu, w2 >> 16};
+                const TilePos pos{lpos, lp};
+                CT out[18];
+                if (MIXED)
+                    tri_forces_mixed(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
+                else
+                    tri_forces_fp64(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
+                const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
+                const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
+                const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
+#pragma unroll
+                for (int s = 0; s < 6; ++s) {
+                    const unsigned q = dq[s];
+                    if (q != 0xFFFFu) {
+                        lcon[3 * q] = out[3 * s];
+                        lcon[3 * q + 1] = out[3 * s + 1];
+                        lcon[3 * q + 2] = out[3 * s + 2];
+                    }
+                }
+            }
+            __syncthreads();
+            // 3. each node adds this pass's entries, in CSR order
+            if ((int)threadIdx.x < nn) {
+                const unsigned short* po = poff + pass * k.tl.poff_stride;
+                ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
+                                 (long long)po[threadIdx.x + 1], fx, fy, fz);
+            }
+            __syncthreads();  // the next pass (or tile) reuses the LDS
+        }
+        // the node's update
         const int n = ch.x + threadIdx.x;
         double fnorm = -1.0, vnorm = -1.0;
         int fidx = INT_MAX, vidx = INT_MAX;
         if (n < ch.y) {
-            const long long base = k.nd.csr_off[ch.x];
-            double fx, fy, fz;
-            ordered_sum3(lds, [](long long e) { return e; }, k.nd.csr_off[n] - base, k.nd.csr_off[n + 1] - base, fx,
-                         fy, fz);
             const double* V = k.b.v[par];
             double* Xo = k.b.x[par ^ 1];
             double* Vo = k.b.v[par ^ 1];
@@ -141,7 +178,8 @@ __global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
             }
             const double lim = sh.lim;
             if (!(fabs(fx) < lim) || !(fabs(fy) < lim) || !(fabs(fz) < lim)) atomicOr(&k.shells[ch.z].failed, 1);
-            double x = X[n], y = X[N + n], z = X[2 * N + n];
+            const int i = threadIdx.x;
+            double x = lpos[3 * i], y = lpos[3 * i + 1], z = lpos[3 * i + 2];
             update(sh, m, fx, vx, x);
             update(sh, m, fy, vy, y);
             update(sh, m, fz, vz, z);
@@ -163,7 +201,7 @@ __global__ __launch_bounds__(256) void tile_step_kernel(TileKArgs k) {
                 k.b.part_i[2 * ci + 1] = i2[1];
             }
         }
-        __syncthreads();  // the next tile reuses the LDS
+        __syncthreads();  // the next tile restages the LDS
     }
 }
 
@@ -193,7 +231,9 @@ void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes
                            ncomp, T, Tp, tris);
 }
 
-int tile_lds_bytes(int max_entries, bool mixed) { return max_entries * 3 * (mixed ? 4 : 8); }
+int tile_lds_bytes(const TileData& tl, bool mixed) {
+    return 24 * tl.max_nodes + tl.max_entries * 3 * (mixed ? 4 : 8);
+}
 
 template <bool MIXED>
 hipError_t raise_lds(int bytes) {
@@ -229,7 +269,7 @@ void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, con
     k.nchunks = nchunks;
     k.shells = shells;
     k.fl = fl;
-    const size_t lds = (size_t)tile_lds_bytes(tl.max_entries, mixed);
+    const size_t lds = (size_t)tile_lds_bytes(tl, mixed);
     if (mixed)
         launch_tile_variant<true>(L, k, lds, fl.check_equil != 0);
     else

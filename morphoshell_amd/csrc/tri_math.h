@@ -159,19 +159,40 @@ __device__ __forceinline__ void load_patch_ids(const TriData& td, unsigned t, un
     for (int s = 0; s < 6; ++s) p[s] = (unsigned)ld(td.pid + s * T, t);
 }
 
-// FP64 force contributions of triangle t (out[3*slot + c]) with patch node
-// ids p; X = positions (3N, SoA with stride N), uniform.
-__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t,
-                                                const unsigned p[6], const double* X, double out[18]) {
+// Position fetchers: pos(s, x, y, z) gives patch node s of the triangle.
+// GlobalPos reads the SoA state (3N, stride N) at global patch ids;
+// TilePos reads the tile's LDS copy (x, y, z per tile-local node).
+struct GlobalPos {
+    const double* X;
+    size_t N;
+    const unsigned* p;
+    __device__ __forceinline__ void operator()(int s, double& x, double& y, double& z) const {
+        x = ld(X, p[s]);
+        y = ld(X + N, p[s]);
+        z = ld(X + 2 * N, p[s]);
+    }
+};
+struct TilePos {
+    const double* lds;        // 3 doubles per tile-local node
+    const unsigned* p;        // tile-local ids
+    __device__ __forceinline__ void operator()(int s, double& x, double& y, double& z) const {
+        const double* q = lds + 3 * p[s];
+        x = q[0];
+        y = q[1];
+        z = q[2];
+    }
+};
+
+// FP64 force contributions of triangle t (out[3*slot + c]); pos fetches its
+// six patch nodes.
+template <class Pos>
+__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
+                                                double out[18]) {
 #pragma clang fp contract(on)
-    const size_t T = (size_t)td.T, N = (size_t)td.N;
+    const size_t T = (size_t)td.T;
     double Px[6], Py[6], Pz[6];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        Px[s] = ld(X, p[s]);
-        Py[s] = ld(X + N, p[s]);
-        Pz[s] = ld(X + 2 * N, p[s]);
-    }
+    for (int s = 0; s < 6; ++s) pos(s, Px[s], Py[s], Pz[s]);
     double Rx[3] = {0.0, 0.0, 0.0}, Ry[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
@@ -260,18 +281,22 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t,
-                                                 const unsigned p[6], const double* X, float out[18]) {
+template <class Pos>
+__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
+                                                 float out[18]) {
 #pragma clang fp contract(on)
-    const size_t T = (size_t)td.T, N = (size_t)td.N;
-    const double x0 = ld(X, p[0]), y0 = ld(X + N, p[0]), z0 = ld(X + 2 * N, p[0]);
+    const size_t T = (size_t)td.T;
+    double x0, y0, z0;
+    pos(0, x0, y0, z0);
     float e[6][3];
     e[0][0] = e[0][1] = e[0][2] = 0.0f;
 #pragma unroll
     for (int s = 1; s < 6; ++s) {
-        e[s][0] = (float)(ld(X, p[s]) - x0);
-        e[s][1] = (float)(ld(X + N, p[s]) - y0);
-        e[s][2] = (float)(ld(X + 2 * N, p[s]) - z0);
+        double x, y, z;
+        pos(s, x, y, z);
+        e[s][0] = (float)(x - x0);
+        e[s][1] = (float)(y - y0);
+        e[s][2] = (float)(z - z0);
     }
     float Rx[3], Ry[3], R2[3][3];
     {
