@@ -509,3 +509,62 @@ def test_autotuner_picks_a_shape_without_changing_results(tmp_path, monkeypatch)
         b.step(20)
         for i in range(2):
             assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0])
+
+
+def _annulus(spacing=0.06, r_in=0.35, seed=5):
+    """A disc with its centre removed (inner boundary, one hole), nodes
+    relabelled to the kept ones."""
+    from morphoshell_amd.meshing import Mesh, grf, isotropic_abar_info, uniform_bbar_info
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    m = grf_disc(spacing, seed=seed, h=0.02, kappa=0.4)
+    c = m.centroids()
+    keep = np.hypot(c[:, 0], c[:, 1]) > r_in
+    tris = m.tris[keep]
+    used = np.unique(tris)
+    remap = np.full(m.num_nodes, -1, np.int64)
+    remap[used] = np.arange(len(used))
+    a = Mesh(points=m.points[used], tris=remap[tris].astype(np.int32))
+    ca = a.centroids()
+    a.abar_info = isotropic_abar_info(1.0 + 0.08 * grf(ca[:, :2], 0.3, seed=seed + 1))
+    a.bbar_info = uniform_bbar_info(a.num_tris, 0.4)
+    a.fill_defaults(h=0.02)
+    cl = np.zeros(a.num_nodes, np.uint8)
+    cl[np.hypot(a.points[:, 0], a.points[:, 1]) > 0.97] = 1  # clamp the outer rim
+    a.constrained = cl
+    return prepare_shell(a, Settings())
+
+
+def test_annulus_matches_oracle_in_both_step_forms():
+    s = _annulus()
+    assert s.info["num_holes"] == 1
+    x, v = deformed_state(s, seed=80)
+    with _ctx(s) as a, _ctx(s) as b:
+        b.set_tuning(node_block=64, fused=1)
+        for c in (a, b):
+            c.set_state(0, x, v)
+            c.step(25)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+        xg = a.get_state(0)[0]
+    xo, vo = x.copy(), v.copy()
+    assert orc.System.from_shell(s).run_steps(xo, vo, 25) == -1
+    assert np.abs(xg - xo).max() <= 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_many_tiny_shells_fused_equals_two_launch(precision):
+    """A ragged batch of 40 shells of 30-400 triangles (most smaller than one
+    tile): the fused step equals the two-launch step bit for bit."""
+    shells = [_shell(0.16 + 0.25 * (i % 5) / 4, seed=i, h=0.02 + 0.005 * (i % 3), kappa=0.3 * (i % 4) - 0.4)
+              for i in range(40)]
+    states = [deformed_state(sh, seed=100 + i) for i, sh in enumerate(shells)]
+    with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        b.set_tuning(node_block=256, fused=1)
+        for i, st in enumerate(states):
+            a.set_state(i, *st)
+            b.set_state(i, *st)
+        ra = a.step(30, equil_every=7, force_tol=1e-30, speed_tol=1e-30)
+        rb = b.step(30, equil_every=7, force_tol=1e-30, speed_tol=1e-30)
+        for i in range(len(shells)):
+            assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
+            assert ra[i]["max_force_ratio"] == rb[i]["max_force_ratio"], i
