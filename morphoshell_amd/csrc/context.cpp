@@ -202,7 +202,7 @@ int build_tiles(ms_ctx* c) {
     free_tiles(c);
     const int T = c->T, N = c->N;
     const int B = c->tune.node_block;
-    const int split = 2 * B;
+    const int split = B;
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
     std::vector<Chunk> nc(c->n_node_chunks);
@@ -212,17 +212,26 @@ int build_tiles(ms_ctx* c) {
     HIPC(c, hipMemcpy(ent.data(), c->csr_ent, ent.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPC(c, hipMemcpy(nc.data(), c->d_node_chunks, nc.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
     const int stride = B + 1;
+    // pass count of the largest tile first (it sizes the offset table)
+    int max_passes = 1;
+    std::vector<std::vector<int>> tile_tris(nc.size());
+    for (size_t ci = 0; ci < nc.size(); ++ci) {
+        std::vector<int>& tl = tile_tris[ci];
+        for (long long e = off[nc[ci].x]; e < off[nc[ci].y]; ++e) tl.push_back(ent[e] / 6);
+        std::sort(tl.begin(), tl.end());
+        tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+        max_passes = std::max(max_passes, (int)((tl.size() + split - 1) / split));
+    }
     std::vector<int> toff(nc.size() + 1, 0), tris, hoff(nc.size() + 1, 0), halo;
-    std::vector<unsigned short> lpid, dst, poff(nc.size() * 2 * (size_t)stride, 0);
+    std::vector<unsigned short> lpid, dst, poff(nc.size() * (size_t)max_passes * stride, 0);
     int max_entries = 0, max_nodes = 0;
-    std::vector<int> tl, hl, cnt0;
+    std::vector<int> hl;
+    std::vector<std::vector<int>> cnt;
     for (size_t ci = 0; ci < nc.size(); ++ci) {
         const int n0 = nc[ci].x, n1 = nc[ci].y, nn = n1 - n0;
         if (nn > B) return fail(c, MS_ERR_ARG, "fused step: a node chunk is larger than the block");
-        tl.clear();
-        for (long long e = off[n0]; e < off[n1]; ++e) tl.push_back(ent[e] / 6);
-        std::sort(tl.begin(), tl.end());
-        tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+        const std::vector<int>& tl = tile_tris[ci];
+        const int npass = (int)((tl.size() + split - 1) / split);
         hl.clear();
         for (int t : tl)
             for (int s = 0; s < 6; ++s) {
@@ -234,27 +243,28 @@ int build_tiles(ms_ctx* c) {
         if (nn + (long long)hl.size() > 0xFFFF)
             return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu reaches too many nodes", ci));
         max_nodes = std::max(max_nodes, nn + (int)hl.size());
-        // entries of each own node in the first pass: those of its triangles
-        // among the tile's first `split` (a prefix of the node's CSR range)
-        const long long t_split = (long long)tl.size() > split ? tl[split] : (long long)INT_MAX + 1;
-        cnt0.assign(nn, 0);
-        unsigned short* p0 = &poff[(2 * ci) * (size_t)stride];
-        unsigned short* p1 = p0 + stride;
-        for (int i = 0; i < nn; ++i) {
-            const long long b = off[n0 + i], e = off[n0 + i + 1];
-            long long k = b;
-            while (k < e && ent[k] / 6 < t_split) ++k;
-            cnt0[i] = (int)(k - b);
-            const long long a0 = p0[i] + (long long)cnt0[i], a1 = p1[i] + (e - b - cnt0[i]);
-            if (a0 > 0xFFFE || a1 > 0xFFFE)
-                return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu has too many patch entries", ci));
-            p0[i + 1] = (unsigned short)a0;
-            p1[i + 1] = (unsigned short)a1;
+        // per node and pass, the number of its entries: an entry's pass is its
+        // triangle's position in the tile list over `split`; along the node's
+        // CSR range (ascending triangles) the pass never decreases
+        cnt.assign(npass, std::vector<int>(nn, 0));
+        for (int i = 0; i < nn; ++i)
+            for (long long e = off[n0 + i]; e < off[n0 + i + 1]; ++e) {
+                const int jj = (int)(std::lower_bound(tl.begin(), tl.end(), ent[e] / 6) - tl.begin());
+                cnt[jj / split][i] += 1;
+            }
+        unsigned short* pbase = &poff[ci * (size_t)max_passes * stride];
+        for (int p = 0; p < npass; ++p) {
+            unsigned short* po = pbase + (size_t)p * stride;
+            for (int i = 0; i < nn; ++i) {
+                const long long a = po[i] + (long long)cnt[p][i];
+                if (a > 0xFFFE) return fail(c, MS_ERR_ARG, fmt("fused step: tile %zu has too many patch entries", ci));
+                po[i + 1] = (unsigned short)a;
+            }
+            max_entries = std::max(max_entries, (int)po[nn]);
         }
-        max_entries = std::max(max_entries, std::max((int)p0[nn], (int)p1[nn]));
         for (size_t jj = 0; jj < tl.size(); ++jj) {
             const int t = tl[jj];
-            const int pass = (int)jj < split ? 0 : 1;
+            const int pass = (int)jj / split;
             tris.push_back(t);
             for (int s = 0; s < 6; ++s) {
                 const int p = pid[(size_t)s * T + t];
@@ -268,8 +278,10 @@ int build_tiles(ms_ctx* c) {
                     const int* it = std::lower_bound(b, e, key);
                     if (it == e || *it != key)
                         return fail(c, MS_ERR_ARG, "fused step: patch-incidence table is inconsistent");
-                    const int rank = (int)(it - b) - (pass ? cnt0[i] : 0);
-                    q = (unsigned short)((pass ? p1[i] : p0[i]) + rank);
+                    int earlier = 0;
+                    for (int pp = 0; pp < pass; ++pp) earlier += cnt[pp][i];
+                    const int rank = (int)(it - b) - earlier;
+                    q = (unsigned short)(pbase[(size_t)pass * stride + i] + rank);
                 } else {
                     l = (unsigned short)(nn + (std::lower_bound(hl.begin(), hl.end(), p) - hl.begin()));
                 }
@@ -284,6 +296,7 @@ int build_tiles(ms_ctx* c) {
     TileData& td = c->tile;
     td.poff_stride = stride;
     td.pass_split = split;
+    td.max_passes = max_passes;
     td.max_nodes = max_nodes;
     td.max_entries = max_entries;
     const int lds64 = tile_lds_bytes(td, false), lds32 = tile_lds_bytes(td, true);
@@ -395,6 +408,9 @@ void params_to_dev(const ms_shell_params& p, ShellDev& S) {
 
 TileData tile_data(const ms_ctx* c) { return c->tile; }
 
+// Some shell uses the LCE dial protocol (its dial info has the 5 LCE columns).
+bool has_lce(const ms_ctx* c) { return c->acols == 5; }
+
 // One step; with the fused tuning the tile launch replaces the triangle and
 // node launches.  timed: events around the two parts (ev[0..2]).
 void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v, bool timed = false) {
@@ -405,12 +421,13 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
         launch_tile_step(node_launch(c), nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks, c->d_shells,
-                         mixed, fl);
+                         mixed, has_lce(c), fl);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                                 c->d_clock, fl);
     } else {
-        launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, true);
+        launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, has_lce(c),
+                          true);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                          c->d_clock, mixed, fl);
@@ -827,7 +844,8 @@ extern "C" int ms_get_precision(const ms_ctx* c) { return c ? c->precision : -1;
 extern "C" int ms_calc_deformation_forces(ms_ctx* c) {
     if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
     const bool mixed = c->precision == MS_MIXED;
-    launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, false);
+    launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed,
+                      has_lce(c), false);
     launch_node_gather(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, mixed);
     HIPC(c, hipGetLastError());
     return MS_OK;

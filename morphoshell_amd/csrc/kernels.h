@@ -57,7 +57,7 @@ struct Launch {
 
 // Triangle launches.  skip_halted: ms_step semantics (halted shells skipped).
 void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                       ShellDev* shells, bool mixed, bool skip_halted);
+                       ShellDev* shells, bool mixed, bool lce, bool skip_halted);
 // Energy densities (dens: 2T, may be null) and per-chunk double-double
 // partial sums of A*W_s and A*W_b into b.part[4*chunk .. +3].
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
@@ -111,10 +111,10 @@ void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
 // halo nodes [halo_off[c], halo_off[c+1]) of its triangles are staged in LDS
 // (tile-local ids: own node n -> n - first, halo node k -> own count + k);
 // lpid[6j + slot] is the tile-local id of a patch node.  The triangles run
-// in two passes (the first pass_split entries, then the rest); dst[6j + slot]
-// is the pass-local LDS position of the patch-incidence entry (j, slot), or
-// 0xFFFF when that node lies outside the tile, and poff[(2c + pass) *
-// poff_stride + i] is the first pass-local position of own node i's entries.
+// in passes of pass_split entries; dst[6j + slot] is the pass-local LDS
+// position of the patch-incidence entry (j, slot), or 0xFFFF when that node
+// lies outside the tile, and poff[(c * max_passes + pass) * poff_stride + i]
+// is the first pass-local position of own node i's entries.
 struct TileData {
     const int* tri_off;            // nchunks + 1
     const int* tris;               // triangle of entry j
@@ -122,9 +122,10 @@ struct TileData {
     const unsigned short* dst;     // 6 per entry
     const int* halo_off;           // nchunks + 1
     const int* halo;               // global ids of halo nodes
-    const unsigned short* poff;    // 2 * poff_stride per tile
+    const unsigned short* poff;    // max_passes * poff_stride per tile
     int poff_stride;               // B + 1
-    int pass_split;                // entries in the first pass (2 B)
+    int pass_split;                // entries per pass (B)
+    int max_passes;
     int max_nodes;                 // max over tiles of own + halo nodes
     int max_entries;               // max over tiles and passes of pass entries
     TriData td;                    // tile-ordered records, td.T = tri_off[nchunks]
@@ -137,8 +138,10 @@ int tile_lds_bytes(const TileData& tl, bool mixed);
 hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // One fused step of every shell not halted (the work of launch_tri_forces +
 // launch_node_step's node kernel), followed by launch_step_bookkeeping.
+// lce: some shell of the context uses the LCE dial protocol (selects the
+// kernel variant that evaluates it).
 void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl);
+                      int nchunks, ShellDev* shells, bool mixed, bool lce, const StepFlags& fl);
 // The per-step bookkeeping launch of launch_node_step (one block).
 void launch_step_bookkeeping(hipStream_t s, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                              ShellDev* shells, int num_shells, ClockDev* clock, const StepFlags& fl);

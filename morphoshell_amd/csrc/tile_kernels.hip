@@ -11,9 +11,9 @@
 //   3. one thread per node sums them in that order -- the order of the
 //      two-launch path, so forces are bit-identical to it -- and applies
 //      damping, clamp, the equilibrium maxima, the blow-up flag and the update.
-// Steps 2-3 run in two passes over the tile's triangles (the first 2B, then
-// the rest) with each node's running sum carried over, which halves the LDS
-// the contributions need.  The per-slot contributions never go to HBM.
+// Steps 2-3 run in passes of B triangles (B = block size) with each node's
+// running sum carried over, so the LDS holds one pass's contributions (about
+// a third of the tile's).  The per-slot contributions never go to HBM.
 
 #include <limits.h>
 
@@ -78,12 +78,12 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
 
 // LDS layout: positions (3 doubles per tile-local node), then contributions
 // (3 CT per pass entry).
-// The mixed kernel is asked for three waves per SIMD (its LDS tile allows
-// them, at the cost of a few spilled registers); the FP64 one is held at two
-// by its LDS tile anyway.
-template <bool MIXED, bool CHECK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void tile_step_kernel(
-    TileKArgs k) {
+// Waves per SIMD asked of the compiler (their LDS tiles allow them): without
+// the LCE protocol 3 in FP64 (168 registers, no spill) and 4 in mixed mode
+// (a few bytes spilled); with it, 3 in mixed mode and what fits in FP64.
+template <bool MIXED, bool CHECK, bool LCE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 1) : (MIXED ? 4 : 3))))
+void tile_step_kernel(TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
     extern __shared__ double smem[];
     double* lpos = smem;
@@ -110,13 +110,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
         }
         __syncthreads();
         const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_off[ci + 1]);
-        const int jm = min(j1, j0 + k.tl.pass_split);
-        const unsigned short* poff = k.tl.poff + (size_t)(2 * ci) * k.tl.poff_stride;
+        const int split = k.tl.pass_split;
+        const int npass = (j1 - j0 + split - 1) / split;
+        const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * k.tl.poff_stride;
         double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll 1
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < npass; ++pass) {
             // 2. this pass's triangles into LDS
-            const int ja = pass ? jm : j0, jb = pass ? j1 : jm;
+            const int ja = j0 + pass * split, jb = min(j1, ja + split);
             for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
                 // opaque to loop strength reduction: the record loads keep one
                 // 32-bit lane offset against scalar stream bases instead of a
@@ -130,9 +131,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
                 const TilePos pos{lpos, lp};
                 CT out[18];
                 if (MIXED)
-                    tri_forces_mixed(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
+                    tri_forces_mixed<LCE>(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
                 else
-                    tri_forces_fp64(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
+                    tri_forces_fp64<LCE>(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
                 const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
                 const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
                 const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
@@ -238,7 +239,10 @@ int tile_lds_bytes(const TileData& tl, bool mixed) {
 template <bool MIXED>
 hipError_t raise_lds(int bytes) {
     if (bytes <= 65536) return hipSuccess;
-    const void* fns[2] = {(const void*)tile_step_kernel<MIXED, false>, (const void*)tile_step_kernel<MIXED, true>};
+    const void* fns[4] = {(const void*)tile_step_kernel<MIXED, false, false>,
+                          (const void*)tile_step_kernel<MIXED, true, false>,
+                          (const void*)tile_step_kernel<MIXED, false, true>,
+                          (const void*)tile_step_kernel<MIXED, true, true>};
     for (const void* f : fns) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -251,16 +255,16 @@ hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed) {
     return e != hipSuccess ? e : raise_lds<true>(lds_bytes_mixed);
 }
 
-template <bool MIXED>
+template <bool MIXED, bool LCE>
 void launch_tile_variant(const Launch& L, const TileKArgs& k, size_t lds, bool check) {
     if (check)
-        hipLaunchKernelGGL((tile_step_kernel<MIXED, true>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, true, LCE>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
     else
-        hipLaunchKernelGGL((tile_step_kernel<MIXED, false>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, false, LCE>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
 }
 
 void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, const StepFlags& fl) {
+                      int nchunks, ShellDev* shells, bool mixed, bool lce, const StepFlags& fl) {
     TileKArgs k{};
     k.nd = nd;
     k.b = b;
@@ -270,10 +274,18 @@ void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, con
     k.shells = shells;
     k.fl = fl;
     const size_t lds = (size_t)tile_lds_bytes(tl, mixed);
-    if (mixed)
-        launch_tile_variant<true>(L, k, lds, fl.check_equil != 0);
-    else
-        launch_tile_variant<false>(L, k, lds, fl.check_equil != 0);
+    const bool check = fl.check_equil != 0;
+    if (mixed) {
+        if (lce)
+            launch_tile_variant<true, true>(L, k, lds, check);
+        else
+            launch_tile_variant<true, false>(L, k, lds, check);
+    } else {
+        if (lce)
+            launch_tile_variant<false, true>(L, k, lds, check);
+        else
+            launch_tile_variant<false, false>(L, k, lds, check);
+    }
 }
 
 }  // namespace ms

@@ -82,7 +82,7 @@ struct TriKArgs {
 };
 
 // ------------------------------------------------------------------ forces
-template <bool MIXED>
+template <bool MIXED, bool LCE>
 __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
@@ -98,13 +98,13 @@ __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
         const GlobalPos pos{X, (size_t)k.td.N, p};
         if (MIXED) {
             float out[18];
-            tri_forces_mixed(k.td, tp, (unsigned)t, pos, out);
+            tri_forces_mixed<LCE>(k.td, tp, (unsigned)t, pos, out);
             float2* dst = reinterpret_cast<float2*>(k.contrib32 + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
         } else {
             double out[18];
-            tri_forces_fp64(k.td, tp, (unsigned)t, pos, out);
+            tri_forces_fp64<LCE>(k.td, tp, (unsigned)t, pos, out);
             double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);
@@ -220,13 +220,21 @@ TriKArgs make_args(const TriData& td, const Buffers& b, const Chunk* chunks, int
 }  // namespace
 
 void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                       ShellDev* shells, bool mixed, bool skip_halted) {
+                       ShellDev* shells, bool mixed, bool lce, bool skip_halted) {
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.skip_halted = skip_halted ? 1 : 0;
-    if (mixed)
-        hipLaunchKernelGGL(tri_forces_kernel<true>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
-    else
-        hipLaunchKernelGGL(tri_forces_kernel<false>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+    const dim3 g(L.grid), bl(L.block);
+    if (mixed) {
+        if (lce)
+            hipLaunchKernelGGL((tri_forces_kernel<true, true>), g, bl, 0, L.stream, k);
+        else
+            hipLaunchKernelGGL((tri_forces_kernel<true, false>), g, bl, 0, L.stream, k);
+    } else {
+        if (lce)
+            hipLaunchKernelGGL((tri_forces_kernel<false, true>), g, bl, 0, L.stream, k);
+        else
+            hipLaunchKernelGGL((tri_forces_kernel<false, false>), g, bl, 0, L.stream, k);
+    }
 }
 
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,

@@ -82,7 +82,10 @@ __device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
 }
 
 // Programmed abar, bbar of triangle t (do_dialling.cpp:83-84 protocol 1,
-// 104-124 protocol 3, 194-228 blend from the ansatz a, b).
+// 104-124 protocol 3, 194-228 blend from the ansatz a, b).  LCE = false
+// compiles only the blend protocol: the LCE branch's pow() costs the hot
+// kernels some 60 registers, so contexts without LCE shells do without it.
+template <bool LCE = true>
 __device__ __forceinline__ void targets(const TriData& td, const TriParams& s, unsigned t, double ab[3],
                                         double bb[3]) {
 #pragma clang fp contract(on)
@@ -98,7 +101,7 @@ __device__ __forceinline__ void targets(const TriData& td, const TriParams& s, u
         for (int k = 0; k < 3; ++k) B0[k] = ld(td.b0 + k * T, t);
     }
     const double fin = s.from_ansatz ? 1.0 : f;
-    if (s.protocol == 3) {
+    if (LCE && s.protocol == 3) {
         const double c2 = ld(td.ainfo, t), cs = ld(td.ainfo + T, t), s2 = ld(td.ainfo + 2 * T, t);
         const double lamf = ld(td.ainfo + 3 * T, t), m2nu = ld(td.ainfo + 4 * T, t);
         const double lam = (1.0 - fin) * 1.0 + fin * lamf;
@@ -185,7 +188,7 @@ struct TilePos {
 
 // FP64 force contributions of triangle t (out[3*slot + c]); pos fetches its
 // six patch nodes.
-template <class Pos>
+template <bool LCE, class Pos>
 __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                 double out[18]) {
 #pragma clang fp contract(on)
@@ -232,7 +235,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
     double ab[3], bb[3];
-    targets(td, sp, t, ab, bb);
+    targets<LCE>(td, sp, t, ab, bb);
     const double P[3] = {ab[2], -ab[1], ab[0]};
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
@@ -281,7 +284,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-template <class Pos>
+template <bool LCE, class Pos>
 __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                  float out[18]) {
 #pragma clang fp contract(on)
@@ -321,7 +324,7 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
         for (int c = 0; c < 3; ++c) R2[kk][c] = acc[c];
     }
     double ab[3], bb[3];
-    targets(td, sp, t, ab, bb);
+    targets<LCE>(td, sp, t, ab, bb);
     // strains: products of float derivatives are exact in double
     const double dRx0 = Rx[0], dRx1 = Rx[1], dRx2 = Rx[2], dRy0 = Ry[0], dRy1 = Ry[1], dRy2 = Ry[2];
     const float Da[3] = {(float)((dRx0 * dRx0 + dRx1 * dRx1 + dRx2 * dRx2) - ab[0]),
