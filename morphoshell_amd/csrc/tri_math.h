@@ -289,47 +289,65 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
                                                  float out[18]) {
 #pragma clang fp contract(on)
     const size_t T = (size_t)td.T;
+    // Edge vectors, the surface derivatives and the strains a - abar, b - bbar
+    // are formed in FP64 (the FP32 patch-fit coefficients are exact in
+    // double).  Rounding the tangent vectors to FP32 before forming the
+    // metric makes a - abar a noisy function of the positions (~1e-7 per
+    // step, uncorrelated between steps); under the weak damping that noise
+    // holds the residual force far above its static error.  The stresses
+    // and the force assembly below run in FP32.
+    float g[4], hc[5][3];
+    g[0] = ld(td.G32 + 1 * T, t);
+    g[1] = ld(td.G32 + 2 * T, t);
+    g[2] = ld(td.G32 + 4 * T, t);
+    g[3] = ld(td.G32 + 5 * T, t);
+#pragma unroll
+    for (int s = 1; s < 6; ++s)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) hc[s - 1][kk] = ld(td.H32 + (s * 3 + kk) * T, t);
+    double ab[3], bb[3];
+    targets<LCE>(td, sp, t, ab, bb);
     double x0, y0, z0;
     pos(0, x0, y0, z0);
-    float e[6][3];
-    e[0][0] = e[0][1] = e[0][2] = 0.0f;
+    // one edge vector at a time into the first and second derivatives
+    double rx[3], ry[3], r2[3][3];
 #pragma unroll
     for (int s = 1; s < 6; ++s) {
         double x, y, z;
         pos(s, x, y, z);
-        e[s][0] = (float)(x - x0);
-        e[s][1] = (float)(y - y0);
-        e[s][2] = (float)(z - z0);
-    }
-    float Rx[3], Ry[3], R2[3][3];
-    {
-        const float g01 = ld(td.G32 + 1 * T, t), g02 = ld(td.G32 + 2 * T, t);
-        const float g11 = ld(td.G32 + 4 * T, t), g12 = ld(td.G32 + 5 * T, t);
+        const double e[3] = {x - x0, y - y0, z - z0};
+        if (s == 1) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            Rx[c] = g01 * e[1][c] + g02 * e[2][c];
-            Ry[c] = g11 * e[1][c] + g12 * e[2][c];
-        }
-    }
+            for (int c = 0; c < 3; ++c) {
+                rx[c] = (double)g[0] * e[c];
+                ry[c] = (double)g[2] * e[c];
+            }
+        } else if (s == 2) {
 #pragma unroll
-    for (int kk = 0; kk < 3; ++kk) {
-        float acc[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 1; s < 6; ++s) {
-            const float hv = ld(td.H32 + (s * 3 + kk) * T, t);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c] += hv * e[s][c];
+            for (int c = 0; c < 3; ++c) {
+                rx[c] += (double)g[1] * e[c];
+                ry[c] += (double)g[3] * e[c];
+            }
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) R2[kk][c] = acc[c];
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (s == 1)
+                    r2[kk][c] = (double)hc[0][kk] * e[c];
+                else
+                    r2[kk][c] += (double)hc[s - 1][kk] * e[c];
+            }
     }
-    double ab[3], bb[3];
-    targets<LCE>(td, sp, t, ab, bb);
-    // strains: products of float derivatives are exact in double
-    const double dRx0 = Rx[0], dRx1 = Rx[1], dRx2 = Rx[2], dRy0 = Ry[0], dRy1 = Ry[1], dRy2 = Ry[2];
-    const float Da[3] = {(float)((dRx0 * dRx0 + dRx1 * dRx1 + dRx2 * dRx2) - ab[0]),
-                         (float)((dRx0 * dRy0 + dRx1 * dRy1 + dRx2 * dRy2) - ab[1]),
-                         (float)((dRy0 * dRy0 + dRy1 * dRy1 + dRy2 * dRy2) - ab[2])};
+    float Rx[3], Ry[3], R2[3][3], Da[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Rx[c] = (float)rx[c];
+        Ry[c] = (float)ry[c];
+    }
+    Da[0] = (float)((rx[0] * rx[0] + rx[1] * rx[1] + rx[2] * rx[2]) - ab[0]);
+    Da[1] = (float)((rx[0] * ry[0] + rx[1] * ry[1] + rx[2] * ry[2]) - ab[1]);
+    Da[2] = (float)((ry[0] * ry[0] + ry[1] * ry[1] + ry[2] * ry[2]) - ab[2]);
     const float m0 = Rx[1] * Ry[2] - Rx[2] * Ry[1];
     const float m1 = Rx[2] * Ry[0] - Rx[0] * Ry[2];
     const float m2 = Rx[0] * Ry[1] - Rx[1] * Ry[0];
@@ -338,8 +356,10 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
     float Db[3];
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
-        const float bk = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
-        Db[kk] = (float)((double)bk - bb[kk]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R2[kk][c] = (float)r2[kk][c];
+        const double bk = -((double)n[0] * r2[kk][0] + (double)n[1] * r2[kk][1] + (double)n[2] * r2[kk][2]);
+        Db[kk] = (float)(bk - bb[kk]);
     }
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
