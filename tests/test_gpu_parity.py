@@ -568,3 +568,65 @@ def test_many_tiny_shells_fused_equals_two_launch(precision):
         for i in range(len(shells)):
             assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
             assert ra[i]["max_force_ratio"] == rb[i]["max_force_ratio"], i
+
+
+_GRF_CHECKS = {}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_grf_disc_checked_at_steps_0_100_5000(precision):
+    """The large smooth-random-field disc (isotropic abar = lambda^2 I,
+    lambda = 1 + 0.1 g, 64-mode field of correlation length 0.2, bbar = 0,
+    h = 0.005) on the same generator at spacing 0.02, run with the fused
+    step from a flat start with z-noise and checked at steps 0, 100, 5000.
+    At every check, against the oracle at the device's own state: forces
+    within 1e-10 of max |F| (FP64) or 1e-5 (mixed), stretch and bend energy
+    within 1e-11 relative (the energy sums are FP64 in both modes).
+    Against the oracle's own trajectory: the FP64 positions agree to 1e-9
+    and the energies to 1e-9 at step 100 (mixed: 1e-4).  By step 5000 the
+    sheet is buckling and the trajectories have parted through the
+    instability (max |dx| ~0.1), so only the total elastic energy is held to
+    10%.  The errors are printed for the record."""
+    from morphoshell_amd import workloads
+    from morphoshell_amd.shell import Context
+
+    sh = workloads.grf_disc(0.02, seed=3)
+    x0, v0 = workloads.flat_noisy_state(sh, seed=5)
+    sysm = orc.System.from_shell(sh)
+    if "oracle" not in _GRF_CHECKS:
+        xo, vo, done, states = x0.copy(), v0.copy(), 0, {}
+        for k in (0, 100, 5000):
+            assert sysm.run_steps(xo, vo, k - done) == -1
+            done = k
+            states[k] = (xo.copy(), sysm.energies(xo, vo))
+        _GRF_CHECKS["oracle"] = states
+    states = _GRF_CHECKS["oracle"]
+    f_tol = 1e-10 if precision == "fp64" else 1e-5
+    with Context(sh, precision=precision) as ctx:
+        ctx.set_tuning(node_block=256, fused=1)
+        ctx.set_state(0, x0, v0)
+        done = 0
+        for k in (0, 100, 5000):
+            ctx.step(k - done)
+            done = k
+            x, v = ctx.get_state(0)
+            ctx.calc_deformation_forces()
+            F = ctx.get_forces(0)
+            Fo = sysm.deformation_forces(x)
+            ef = np.abs(F - Fo).max() / np.abs(Fo).max()
+            E = ctx.calc_energies(0)
+            Es = sysm.energies(x, v)
+            es = np.abs(E[:2] - Es[:2]) / np.abs(Es[:2])
+            xo, Eo = states[k]
+            et = np.abs(E[:2] - Eo[:2]) / np.abs(Eo[:2])
+            dx = np.abs(x - xo).max()
+            print(f"{precision} step {k}: force rel err {ef:.2e}, energy rel err at this state {es.max():.2e}; "
+                  f"against the oracle trajectory: stretch/bend energy {et[0]:.2e}/{et[1]:.2e}, max |dx| {dx:.2e}")
+            assert ef <= f_tol
+            assert (es <= 1e-11).all()
+            if k <= 100:
+                assert (et <= (1e-9 if precision == "fp64" else 1e-4)).all()
+                if precision == "fp64":
+                    assert dx <= 1e-9
+            else:
+                assert abs((E[0] + E[1]) - (Eo[0] + Eo[1])) <= 0.1 * (Eo[0] + Eo[1])
