@@ -353,6 +353,10 @@ int ms_synchronize(ms_ctx* ctx);
  * the average kernel time. */
 int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms);
 
+/* Measurement aid: reps launches of a read-only stream over nbytes (the
+ * ceiling of a kernel that mostly reads).  out_ms as above. */
+int ms_calibration_read(int64_t nbytes, int32_t reps, double* out_ms);
+
 /* ====================================================== learned warm start
  * A mesh message-passing network that predicts near-equilibrium node
  * positions from the flat reference mesh and the programmed metric.  It is

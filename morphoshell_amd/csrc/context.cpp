@@ -62,6 +62,7 @@ struct ms_ctx {
     int64_t tile_bytes = 0;          // metadata
     int64_t tile_rec_bytes = 0;      // record copies
     int acols = 3;
+    bool uni_data = true;            // see uni_data()
 
     // graphs
     hipGraphExec_t graph_exec = nullptr;
@@ -411,6 +412,10 @@ TileData tile_data(const ms_ctx* c) { return c->tile; }
 // Some shell uses the LCE dial protocol (its dial info has the 5 LCE columns).
 bool has_lce(const ms_ctx* c) { return c->acols == 5; }
 
+// Every shell's thickness, shear modulus and (blend protocol) bbar_info are
+// uniform over its triangles (set at creation; these arrays never change).
+bool uni_data(const ms_ctx* c) { return c->uni_data; }
+
 // One step; with the fused tuning the tile launch replaces the triangle and
 // node launches.  timed: events around the two parts (ev[0..2]).
 void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v, bool timed = false) {
@@ -421,13 +426,13 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
         launch_tile_step(node_launch(c), nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks, c->d_shells,
-                         mixed, has_lce(c), fl);
+                         mixed, has_lce(c), uni_data(c), fl);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                                 c->d_clock, fl);
     } else {
         launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, has_lce(c),
-                          true);
+                          uni_data(c), true);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                          c->d_clock, mixed, fl);
@@ -554,6 +559,21 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
         S.has_a0 = d.a0 ? 1 : 0;
         S.has_b0 = d.b0 ? 1 : 0;
         const bool lce = S.protocol == 3;
+        {
+            // uniform per-triangle data, compared bit for bit
+            auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+            bool uh = d.num_tris > 0, ub = d.num_tris > 0 && !lce;
+            for (int t = 1; t < d.num_tris && (uh || ub); ++t) {
+                uh = uh && same(d.thickness[t], d.thickness[0]) && same(d.shear_modulus[t], d.shear_modulus[0]);
+                for (int k = 0; k < 3 && ub; ++k) ub = same(d.bbar_info[(size_t)t * 3 + k], d.bbar_info[k]);
+            }
+            S.uni_hmu = uh ? 1 : 0;
+            S.uni_b = ub ? 1 : 0;
+            if (!uh || !(ub || lce)) c->uni_data = false;
+            S.h_u = uh ? d.thickness[0] : 0.0;
+            S.mu_u = uh ? d.shear_modulus[0] : 0.0;
+            for (int k = 0; k < 3; ++k) S.b_u[k] = ub ? d.bbar_info[k] : 0.0;
+        }
         for (int t = 0; t < d.num_tris; ++t) {
             const size_t g = (size_t)tb + t;
             for (int k = 0; k < 6; ++k) pid[k * (size_t)T + g] = d.patch_ids[6 * (size_t)t + k] + nb;
@@ -845,7 +865,7 @@ extern "C" int ms_calc_deformation_forces(ms_ctx* c) {
     if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
     const bool mixed = c->precision == MS_MIXED;
     launch_tri_forces(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed,
-                      has_lce(c), false);
+                      has_lce(c), uni_data(c), false);
     launch_node_gather(node_launch(c), node_data(c), c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, mixed);
     HIPC(c, hipGetLastError());
     return MS_OK;
@@ -1285,25 +1305,35 @@ extern "C" int ms_synchronize(ms_ctx* c) {
     return MS_OK;
 }
 
-extern "C" int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms) {
+namespace {
+// reps timed launches of the copy (read = false) or read-only (read = true)
+// stream over nbytes
+int calibration_stream(int64_t nbytes, int32_t reps, double* out_ms, bool read, const char* name) {
     if (nbytes <= 0 || nbytes % 16 != 0 || reps <= 0)
-        return fail(nullptr, MS_ERR_ARG, "ms_calibration_copy: nbytes must be a positive multiple of 16");
+        return fail(nullptr, MS_ERR_ARG, fmt("%s: nbytes must be a positive multiple of 16", name));
     void *a = nullptr, *b = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t bbytes = read ? (size_t)read_out_floats() * sizeof(float) : (size_t)nbytes;
     int rc = MS_OK;
-    if (hipMalloc(&a, (size_t)nbytes) != hipSuccess || hipMalloc(&b, (size_t)nbytes) != hipSuccess ||
+    if (hipMalloc(&a, (size_t)nbytes) != hipSuccess || hipMalloc(&b, bbytes) != hipSuccess ||
         hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess) {
-        rc = fail(nullptr, MS_ERR_HIP, "ms_calibration_copy: allocation failed");
+        rc = fail(nullptr, MS_ERR_HIP, fmt("%s: allocation failed", name));
     } else {
+        auto launch = [&] {
+            if (read)
+                launch_read(s, a, (float*)b, nbytes);
+            else
+                launch_copy(s, a, b, nbytes);
+        };
         hipMemsetAsync(a, 0, (size_t)nbytes, s);
-        launch_copy(s, a, b, nbytes);  // warm
+        launch();  // warm
         hipEventRecord(e0, s);
-        for (int i = 0; i < reps; ++i) launch_copy(s, a, b, nbytes);
+        for (int i = 0; i < reps; ++i) launch();
         hipEventRecord(e1, s);
         if (hipEventSynchronize(e1) != hipSuccess) {
-            rc = fail(nullptr, MS_ERR_HIP, "ms_calibration_copy: kernel failed");
+            rc = fail(nullptr, MS_ERR_HIP, fmt("%s: kernel failed", name));
         } else if (out_ms) {
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
@@ -1316,6 +1346,15 @@ extern "C" int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms)
     if (a) hipFree(a);
     if (b) hipFree(b);
     return rc;
+}
+}  // namespace
+
+extern "C" int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms) {
+    return calibration_stream(nbytes, reps, out_ms, false, "ms_calibration_copy");
+}
+
+extern "C" int ms_calibration_read(int64_t nbytes, int32_t reps, double* out_ms) {
+    return calibration_stream(nbytes, reps, out_ms, true, "ms_calibration_read");
 }
 
 extern "C" int ms_last_step_timing(const ms_ctx* c, double out[3]) {

@@ -68,6 +68,13 @@ struct ShellDev {
     double f_targets;      // factor abar/bbar were last dialled at (do_dialling runs only
                            // while dialling or at step 0, main.cpp:334-353)
     long long own_steps;   // steps this shell has taken (halted shells do not count)
+    // Per-shell uniform triangle data (bitwise equal over the shell's
+    // triangles): the force kernels take them from here instead of streaming
+    // the per-triangle arrays.  uni_b is set only for blend-protocol shells.
+    int uni_hmu;           // thickness and shear modulus uniform: h_u, mu_u
+    int uni_b;             // bbar_info uniform: b_u
+    double h_u, mu_u;
+    double b_u[3];
 };
 
 // Dial factor a step uses (main.cpp:286-288, 483-485).

@@ -57,7 +57,7 @@ struct Launch {
 
 // Triangle launches.  skip_halted: ms_step semantics (halted shells skipped).
 void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                       ShellDev* shells, bool mixed, bool lce, bool skip_halted);
+                       ShellDev* shells, bool mixed, bool lce, bool uni_data, bool skip_halted);
 // Energy densities (dens: 2T, may be null) and per-chunk double-double
 // partial sums of A*W_s and A*W_b into b.part[4*chunk .. +3].
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
@@ -102,6 +102,9 @@ void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* s
                             int ncomp, double* out);
 // Plain 16-byte-vector copy of nbytes (traffic-counter calibration).
 void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
+// Read-only stream of nbytes; out holds read_out_floats() floats.
+void launch_read(hipStream_t s, const void* src, float* out, long long nbytes);
+long long read_out_floats();
 
 // Fused owner-computes step.  Tile c = node chunk c (at most B = block size
 // nodes).  Entries j in [tri_off[c], tri_off[c+1]) are the triangles
@@ -139,9 +142,11 @@ hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // One fused step of every shell not halted (the work of launch_tri_forces +
 // launch_node_step's node kernel), followed by launch_step_bookkeeping.
 // lce: some shell of the context uses the LCE dial protocol (selects the
-// kernel variant that evaluates it).
+// kernel variant that evaluates it).  uni_data: every shell has uniform
+// thickness, shear modulus and (blend protocol) bbar_info (ShellDev h_u, mu_u,
+// b_u), so the variant that reads them from the shell record runs.
 void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, bool lce, const StepFlags& fl);
+                      int nchunks, ShellDev* shells, bool mixed, bool lce, bool uni_data, const StepFlags& fl);
 // The per-step bookkeeping launch of launch_node_step (one block).
 void launch_step_bookkeeping(hipStream_t s, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                              ShellDev* shells, int num_shells, ClockDev* clock, const StepFlags& fl);

@@ -76,20 +76,25 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
     __syncthreads();
 }
 
-// LDS layout: positions (3 doubles per tile-local node), then contributions
-// (3 CT per pass entry).
+// LDS layout: positions (3 doubles per tile-local node), in mixed mode the
+// nodes' running force sums (3 doubles per own node, carried from pass to
+// pass in LDS so that they hold no registers while triangles are evaluated;
+// the FP64 kernel keeps them in registers, where its larger LDS tile would
+// otherwise cost a workgroup per CU), then contributions (3 CT per pass
+// entry).
 // Waves per SIMD asked of the compiler (their LDS tiles allow them): without
 // the LCE protocol 3 in FP64 (168 registers, no spill) and 4 in mixed mode
 // (a few bytes spilled); with it, 3 in mixed mode and what fits in FP64.
-template <bool MIXED, bool CHECK, bool LCE>
+template <bool MIXED, bool CHECK, bool LCE, bool UNI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 1) : (MIXED ? 4 : 3))))
 void tile_step_kernel(TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
     extern __shared__ double smem[];
-    double* lpos = smem;
-    CT* lcon = reinterpret_cast<CT*>(smem + 3 * (size_t)k.tl.max_nodes);
     const int N = k.nd.N;
     const int B = blockDim.x;
+    double* lpos = smem;
+    double* lsum = smem + 3 * (size_t)k.tl.max_nodes;
+    CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)B : lsum);
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
         const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
@@ -98,7 +103,7 @@ void tile_step_kernel(TileKArgs k) {
         const bool do_check = CHECK || uni(sh.check_due);
         const int par = uni(sh.par);
         const double* X = par ? k.b.x[1] : k.b.x[0];
-        const TriParams tp = tri_params(sh);
+        const TriParams tp = tri_params<UNI>(sh);
         const int nn = ch.y - ch.x;
         const int h0 = uni(k.tl.halo_off[ci]), h1 = uni(k.tl.halo_off[ci + 1]);
         // 1. positions of the tile's nodes and its halo nodes
@@ -131,9 +136,9 @@ void tile_step_kernel(TileKArgs k) {
                 const TilePos pos{lpos, lp};
                 CT out[18];
                 if (MIXED)
-                    tri_forces_mixed<LCE>(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
+                    tri_forces_mixed<LCE, UNI>(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
                 else
-                    tri_forces_fp64<LCE>(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
+                    tri_forces_fp64<LCE, UNI, false>(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
                 const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
                 const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
                 const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
@@ -151,10 +156,30 @@ void tile_step_kernel(TileKArgs k) {
             // 3. each node adds this pass's entries, in CSR order
             if ((int)threadIdx.x < nn) {
                 const unsigned short* po = poff + pass * k.tl.poff_stride;
-                ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
-                                 (long long)po[threadIdx.x + 1], fx, fy, fz);
+                if (MIXED) {
+                    double* ls = lsum + 3 * threadIdx.x;
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    if (pass > 0) {
+                        sx = ls[0];
+                        sy = ls[1];
+                        sz = ls[2];
+                    }
+                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
+                                     (long long)po[threadIdx.x + 1], sx, sy, sz);
+                    ls[0] = sx;
+                    ls[1] = sy;
+                    ls[2] = sz;
+                } else {
+                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
+                                     (long long)po[threadIdx.x + 1], fx, fy, fz);
+                }
             }
             __syncthreads();  // the next pass (or tile) reuses the LDS
+        }
+        if (MIXED && (int)threadIdx.x < nn && npass > 0) {
+            fx = lsum[3 * threadIdx.x];
+            fy = lsum[3 * threadIdx.x + 1];
+            fz = lsum[3 * threadIdx.x + 2];
         }
         // the node's update
         const int n = ch.x + threadIdx.x;
@@ -233,16 +258,20 @@ void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes
 }
 
 int tile_lds_bytes(const TileData& tl, bool mixed) {
-    return 24 * tl.max_nodes + tl.max_entries * 3 * (mixed ? 4 : 8);
+    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8);
 }
 
 template <bool MIXED>
 hipError_t raise_lds(int bytes) {
     if (bytes <= 65536) return hipSuccess;
-    const void* fns[4] = {(const void*)tile_step_kernel<MIXED, false, false>,
-                          (const void*)tile_step_kernel<MIXED, true, false>,
-                          (const void*)tile_step_kernel<MIXED, false, true>,
-                          (const void*)tile_step_kernel<MIXED, true, true>};
+    const void* fns[8] = {(const void*)tile_step_kernel<MIXED, false, false, false>,
+                          (const void*)tile_step_kernel<MIXED, true, false, false>,
+                          (const void*)tile_step_kernel<MIXED, false, true, false>,
+                          (const void*)tile_step_kernel<MIXED, true, true, false>,
+                          (const void*)tile_step_kernel<MIXED, false, false, true>,
+                          (const void*)tile_step_kernel<MIXED, true, false, true>,
+                          (const void*)tile_step_kernel<MIXED, false, true, true>,
+                          (const void*)tile_step_kernel<MIXED, true, true, true>};
     for (const void* f : fns) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -255,16 +284,16 @@ hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed) {
     return e != hipSuccess ? e : raise_lds<true>(lds_bytes_mixed);
 }
 
-template <bool MIXED, bool LCE>
+template <bool MIXED, bool LCE, bool UNI>
 void launch_tile_variant(const Launch& L, const TileKArgs& k, size_t lds, bool check) {
     if (check)
-        hipLaunchKernelGGL((tile_step_kernel<MIXED, true, LCE>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, true, LCE, UNI>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
     else
-        hipLaunchKernelGGL((tile_step_kernel<MIXED, false, LCE>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
+        hipLaunchKernelGGL((tile_step_kernel<MIXED, false, LCE, UNI>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
 }
 
 void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, bool lce, const StepFlags& fl) {
+                      int nchunks, ShellDev* shells, bool mixed, bool lce, bool uni_data, const StepFlags& fl) {
     TileKArgs k{};
     k.nd = nd;
     k.b = b;
@@ -275,17 +304,22 @@ void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, con
     k.fl = fl;
     const size_t lds = (size_t)tile_lds_bytes(tl, mixed);
     const bool check = fl.check_equil != 0;
-    if (mixed) {
-        if (lce)
-            launch_tile_variant<true, true>(L, k, lds, check);
-        else
-            launch_tile_variant<true, false>(L, k, lds, check);
-    } else {
-        if (lce)
-            launch_tile_variant<false, true>(L, k, lds, check);
-        else
-            launch_tile_variant<false, false>(L, k, lds, check);
+    const int v = (mixed ? 4 : 0) | (lce ? 2 : 0) | (uni_data ? 1 : 0);
+#define MS_TILE_CASE(M, C, U)                                \
+    case (M ? 4 : 0) | (C ? 2 : 0) | (U ? 1 : 0):            \
+        launch_tile_variant<M, C, U>(L, k, lds, check); \
+        break;
+    switch (v) {
+        MS_TILE_CASE(false, false, false)
+        MS_TILE_CASE(false, false, true)
+        MS_TILE_CASE(false, true, false)
+        MS_TILE_CASE(false, true, true)
+        MS_TILE_CASE(true, false, false)
+        MS_TILE_CASE(true, false, true)
+        MS_TILE_CASE(true, true, false)
+        MS_TILE_CASE(true, true, true)
     }
+#undef MS_TILE_CASE
 }
 
 }  // namespace ms

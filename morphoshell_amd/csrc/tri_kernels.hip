@@ -82,14 +82,14 @@ struct TriKArgs {
 };
 
 // ------------------------------------------------------------------ forces
-template <bool MIXED, bool LCE>
+template <bool MIXED, bool LCE, bool UNI>
 __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
         const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
         const ShellDev& sh = k.shells[ch.z];
         if (k.skip_halted && uni(sh.halted)) continue;  // block-uniform
-        const TriParams tp = tri_params(sh);
+        const TriParams tp = tri_params<UNI>(sh);
         const double* X = uni(sh.par) ? k.pos1 : k.pos;
         const int t = ch.x + threadIdx.x;
         if (t >= ch.y) continue;
@@ -98,13 +98,13 @@ __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
         const GlobalPos pos{X, (size_t)k.td.N, p};
         if (MIXED) {
             float out[18];
-            tri_forces_mixed<LCE>(k.td, tp, (unsigned)t, pos, out);
+            tri_forces_mixed<LCE, UNI>(k.td, tp, (unsigned)t, pos, out);
             float2* dst = reinterpret_cast<float2*>(k.contrib32 + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
         } else {
             double out[18];
-            tri_forces_fp64<LCE>(k.td, tp, (unsigned)t, pos, out);
+            tri_forces_fp64<LCE, UNI, true>(k.td, tp, (unsigned)t, pos, out);
             double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);
@@ -220,21 +220,26 @@ TriKArgs make_args(const TriData& td, const Buffers& b, const Chunk* chunks, int
 }  // namespace
 
 void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                       ShellDev* shells, bool mixed, bool lce, bool skip_halted) {
+                       ShellDev* shells, bool mixed, bool lce, bool uni_data, bool skip_halted) {
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.skip_halted = skip_halted ? 1 : 0;
     const dim3 g(L.grid), bl(L.block);
-    if (mixed) {
-        if (lce)
-            hipLaunchKernelGGL((tri_forces_kernel<true, true>), g, bl, 0, L.stream, k);
-        else
-            hipLaunchKernelGGL((tri_forces_kernel<true, false>), g, bl, 0, L.stream, k);
-    } else {
-        if (lce)
-            hipLaunchKernelGGL((tri_forces_kernel<false, true>), g, bl, 0, L.stream, k);
-        else
-            hipLaunchKernelGGL((tri_forces_kernel<false, false>), g, bl, 0, L.stream, k);
+    const int v = (mixed ? 4 : 0) | (lce ? 2 : 0) | (uni_data ? 1 : 0);
+#define MS_TRI_CASE(M, C, U)                                                             \
+    case (M ? 4 : 0) | (C ? 2 : 0) | (U ? 1 : 0):                                        \
+        hipLaunchKernelGGL((tri_forces_kernel<M, C, U>), g, bl, 0, L.stream, k); \
+        break;
+    switch (v) {
+        MS_TRI_CASE(false, false, false)
+        MS_TRI_CASE(false, false, true)
+        MS_TRI_CASE(false, true, false)
+        MS_TRI_CASE(false, true, true)
+        MS_TRI_CASE(true, false, false)
+        MS_TRI_CASE(true, false, true)
+        MS_TRI_CASE(true, true, false)
+        MS_TRI_CASE(true, true, true)
     }
+#undef MS_TRI_CASE
 }
 
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,

@@ -61,6 +61,7 @@ __device__ __forceinline__ E ld(const E* base, unsigned i) {
 struct TriParams {
     double nu, gamma, f;  // Poisson ratio, stiffening scale, dial factor
     int protocol, from_ansatz, has_a0, has_b0;
+    double h, mu, b[3];   // uniform h, mu, bbar_info (UNI kernels only)
 };
 
 // Factor the programmed metric is evaluated at: under the device-run protocol
@@ -69,6 +70,11 @@ __device__ __forceinline__ double dial_factor(const ShellDev& s) {
     return s.auto_protocol ? s.f_targets : shell_dial_factor(s);
 }
 
+// UNI: every shell of the context has uniform thickness, shear modulus and
+// (blend protocol) bbar_info, which then come from the shell record; the
+// per-triangle streams are not read.  A compile-time choice: a runtime switch
+// keeps both the uniform values and the stream bases live.
+template <bool UNI = false>
 __device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
     TriParams p;
     p.nu = uni(s.nu);
@@ -78,32 +84,62 @@ __device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
     p.from_ansatz = uni(s.from_ansatz);
     p.has_a0 = uni(s.has_a0);
     p.has_b0 = uni(s.has_b0);
+    if (UNI) {
+        p.h = uni(s.h_u);
+        p.mu = uni(s.mu_u);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p.b[k] = uni(s.b_u[k]);
+    }
     return p;
 }
 
-// Programmed abar, bbar of triangle t (do_dialling.cpp:83-84 protocol 1,
-// 104-124 protocol 3, 194-228 blend from the ansatz a, b).  LCE = false
-// compiles only the blend protocol: the LCE branch's pow() costs the hot
-// kernels some 60 registers, so contexts without LCE shells do without it.
-template <bool LCE = true>
-__device__ __forceinline__ void targets(const TriData& td, const TriParams& s, unsigned t, double ab[3],
-                                        double bb[3]) {
-#pragma clang fp contract(on)
+// The per-triangle dial data of triangle t: dial info (blend: abar_info and
+// bbar_info; LCE: c2, cs, s2, lambda, -2nu) and the ansatz a0, b0 when
+// dialling from it.  The force kernels load it next to the patch
+// coefficients, ahead of every branch and use, so that a triangle's record
+// costs one memory round trip.
+struct DialRec {
+    double ai[5], bi[3], A0[3], B0[3];
+};
+
+template <bool LCE, bool UNI>
+__device__ __forceinline__ void load_dial(const TriData& td, const TriParams& s, unsigned t, DialRec& d) {
     const size_t T = (size_t)td.T;
-    const double f = s.f;
-    double A0[3] = {1.0, 0.0, 1.0}, B0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d.ai[k] = ld(td.ainfo + k * T, t);
+    if (LCE && s.protocol == 3) {
+        d.ai[3] = ld(td.ainfo + 3 * T, t);
+        d.ai[4] = ld(td.ainfo + 4 * T, t);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d.bi[k] = UNI ? s.b[k] : ld(td.binfo + k * T, t);
+    }
+    d.A0[0] = 1.0;
+    d.A0[1] = 0.0;
+    d.A0[2] = 1.0;
+    d.B0[0] = d.B0[1] = d.B0[2] = 0.0;
     if (s.has_a0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) A0[k] = ld(td.a0 + k * T, t);
+        for (int k = 0; k < 3; ++k) d.A0[k] = ld(td.a0 + k * T, t);
     }
     if (s.has_b0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) B0[k] = ld(td.b0 + k * T, t);
+        for (int k = 0; k < 3; ++k) d.B0[k] = ld(td.b0 + k * T, t);
     }
+}
+
+// Programmed abar, bbar from the dial data (do_dialling.cpp:83-84 protocol 1,
+// 104-124 protocol 3, 194-228 blend from the ansatz a, b).  LCE = false
+// compiles only the blend protocol: the LCE branch's pow() costs the hot
+// kernels some 60 registers, so contexts without LCE shells do without it.
+template <bool LCE>
+__device__ __forceinline__ void dial_targets(const DialRec& d, const TriParams& s, double ab[3], double bb[3]) {
+#pragma clang fp contract(on)
+    const double f = s.f;
     const double fin = s.from_ansatz ? 1.0 : f;
     if (LCE && s.protocol == 3) {
-        const double c2 = ld(td.ainfo, t), cs = ld(td.ainfo + T, t), s2 = ld(td.ainfo + 2 * T, t);
-        const double lamf = ld(td.ainfo + 3 * T, t), m2nu = ld(td.ainfo + 4 * T, t);
+        const double c2 = d.ai[0], cs = d.ai[1], s2 = d.ai[2];
+        const double lamf = d.ai[3], m2nu = d.ai[4];
         const double lam = (1.0 - fin) * 1.0 + fin * lamf;
         const double l2 = lam * lam, lp = pow(lam, m2nu);
         ab[0] = l2 * c2 + lp * s2;
@@ -113,17 +149,26 @@ __device__ __forceinline__ void targets(const TriData& td, const TriParams& s, u
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            ab[k] = (1.0 - fin) * A0[k] + fin * ld(td.ainfo + k * T, t);
-            bb[k] = (1.0 - fin) * B0[k] + fin * ld(td.binfo + k * T, t);
+            ab[k] = (1.0 - fin) * d.A0[k] + fin * d.ai[k];
+            bb[k] = (1.0 - fin) * d.B0[k] + fin * d.bi[k];
         }
     }
     if (s.from_ansatz) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            ab[k] = (1.0 - f) * A0[k] + f * ab[k];
-            bb[k] = (1.0 - f) * B0[k] + f * bb[k];
+            ab[k] = (1.0 - f) * d.A0[k] + f * ab[k];
+            bb[k] = (1.0 - f) * d.B0[k] + f * bb[k];
         }
     }
+}
+
+// Programmed abar, bbar of triangle t at the shell's dial factor.
+template <bool LCE = true, bool UNI = false>
+__device__ __forceinline__ void targets(const TriData& td, const TriParams& s, unsigned t, double ab[3],
+                                        double bb[3]) {
+    DialRec d;
+    load_dial<LCE, UNI>(td, s, t, d);
+    dial_targets<LCE>(d, s, ab, bb);
 }
 
 // Phi(D) and its stored-component gradient (dPhi/dD00, dPhi/dD01 counted
@@ -187,19 +232,36 @@ struct TilePos {
 };
 
 // FP64 force contributions of triangle t (out[3*slot + c]); pos fetches its
-// six patch nodes.
-template <bool LCE, class Pos>
+// six patch nodes.  EARLY issues the whole record (patch coefficients, dial
+// data, h, mu, area) before any use, one memory round trip per triangle:
+// faster in the triangle launch; the tile launch, held at 168 registers,
+// keeps more in flight with the loads at their uses.  Same arithmetic.
+template <bool LCE, bool UNI, bool EARLY, class Pos>
 __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                 double out[18]) {
 #pragma clang fp contract(on)
     const size_t T = (size_t)td.T;
+    double gc[6], hc[18], h, mu, A;
+    DialRec dr;
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gc[k] = ld(td.G + k * T, t);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) hc[k] = ld(td.H + k * T, t);
+        h = UNI ? sp.h : ld(td.h, t);
+        mu = UNI ? sp.mu : ld(td.mu, t);
+        A = ld(td.area, t);
+        load_dial<LCE, UNI>(td, sp, t, dr);
+    }
+    const double nu = sp.nu;
     double Px[6], Py[6], Pz[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) pos(s, Px[s], Py[s], Pz[s]);
     double Rx[3] = {0.0, 0.0, 0.0}, Ry[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-        const double g0 = ld(td.G + v * T, t), g1 = ld(td.G + (3 + v) * T, t);
+        const double g0 = EARLY ? gc[v] : ld(td.G + v * T, t);
+        const double g1 = EARLY ? gc[3 + v] : ld(td.G + (3 + v) * T, t);
         Rx[0] += g0 * Px[v];
         Rx[1] += g0 * Py[v];
         Rx[2] += g0 * Pz[v];
@@ -213,7 +275,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
         double sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const double hv = ld(td.H + (s * 3 + kk) * T, t);
+            const double hv = EARLY ? hc[s * 3 + kk] : ld(td.H + (s * 3 + kk) * T, t);
             sx += hv * Px[s];
             sy += hv * Py[s];
             sz += hv * Pz[s];
@@ -235,17 +297,22 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
     double ab[3], bb[3];
-    targets<LCE>(td, sp, t, ab, bb);
+    if constexpr (EARLY) {
+        dial_targets<LCE>(dr, sp, ab, bb);
+    } else {
+        targets<LCE, UNI>(td, sp, t, ab, bb);
+        h = UNI ? sp.h : ld(td.h, t);
+        mu = UNI ? sp.mu : ld(td.mu, t);
+        A = ld(td.area, t);
+    }
     const double P[3] = {ab[2], -ab[1], ab[0]};
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
-    const double h = ld(td.h, t), mu = ld(td.mu, t), nu = sp.nu;
     const double cs = h * mu / (1.0 - nu) * idet32;
     const double cb = mu * h * h * h / (12.0 * (1.0 - nu)) * idet32;
     const double Da[3] = {a0 - ab[0], a1 - ab[1], a2 - ab[2]};
     const double Db[3] = {b[0] - bb[0], b[1] - bb[1], b[2] - bb[2]};
     const double scale = sp.gamma * mu * h;
-    const double A = ld(td.area, t);
     double ga[3], gbv[3];
     phi_grad(nu, P, Da, ga);
     const double Wb0 = cb * phi_grad(nu, P, Db, gbv);
@@ -271,12 +338,17 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const double ws = ld(td.H + (s * 3) * T, t) * Wb[0] + ld(td.H + (s * 3 + 1) * T, t) * Wb[1] +
-                          ld(td.H + (s * 3 + 2) * T, t) * Wb[2];
+        const double ws = EARLY ? hc[s * 3] * Wb[0] + hc[s * 3 + 1] * Wb[1] + hc[s * 3 + 2] * Wb[2]
+                                : ld(td.H + (s * 3) * T, t) * Wb[0] + ld(td.H + (s * 3 + 1) * T, t) * Wb[1] +
+                                      ld(td.H + (s * 3 + 2) * T, t) * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double f = ws * An[c];
-            if (s < 3) f -= ld(td.G + s * T, t) * Agx[c] + ld(td.G + (3 + s) * T, t) * Agy[c];
+            if (s < 3) {
+                const double gx = EARLY ? gc[s] : ld(td.G + s * T, t);
+                const double gy = EARLY ? gc[3 + s] : ld(td.G + (3 + s) * T, t);
+                f -= gx * Agx[c] + gy * Agy[c];
+            }
             out[3 * s + c] = f;
         }
     }
@@ -284,7 +356,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-template <bool LCE, class Pos>
+template <bool LCE, bool UNI, class Pos>
 __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                  float out[18]) {
 #pragma clang fp contract(on)
@@ -305,8 +377,11 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
     for (int s = 1; s < 6; ++s)
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) hc[s - 1][kk] = ld(td.H32 + (s * 3 + kk) * T, t);
-    double ab[3], bb[3];
-    targets<LCE>(td, sp, t, ab, bb);
+    const float h = UNI ? (float)sp.h : ld(td.hmuA32, t);
+    const float mu = UNI ? (float)sp.mu : ld(td.hmuA32 + T, t);
+    const float A = ld(td.hmuA32 + 2 * T, t);
+    DialRec dr;
+    load_dial<LCE, UNI>(td, sp, t, dr);
     double x0, y0, z0;
     pos(0, x0, y0, z0);
     // one edge vector at a time into the first and second derivatives
@@ -339,6 +414,8 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
                     r2[kk][c] += (double)hc[s - 1][kk] * e[c];
             }
     }
+    double ab[3], bb[3];
+    dial_targets<LCE>(dr, sp, ab, bb);
     float Rx[3], Ry[3], R2[3][3], Da[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -363,7 +440,6 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
     }
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
-    const float h = ld(td.hmuA32, t), mu = ld(td.hmuA32 + T, t), A = ld(td.hmuA32 + 2 * T, t);
     const float nu = (float)sp.nu;
     const float cs = (float)((double)h * (double)mu / (1.0 - sp.nu) * idet32);
     const float cb = (float)((double)mu * h * h * h / (12.0 * (1.0 - sp.nu)) * idet32);

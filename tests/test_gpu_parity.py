@@ -630,3 +630,48 @@ def test_grf_disc_checked_at_steps_0_100_5000(precision):
                     assert dx <= 1e-9
             else:
                 assert abs((E[0] + E[1]) - (Eo[0] + Eo[1])) <= 0.1 * (Eo[0] + Eo[1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_per_triangle_thickness_modulus_and_bbar(precision):
+    """A shell with per-triangle thickness, shear modulus and bbar next to one
+    whose values are uniform: the kernels that stream these arrays run (a
+    context takes them from the shell record only when every shell is
+    uniform), both step forms agree bit for bit, and FP64 forces match the
+    oracle to 1e-10 of max |F|; the uniform shell alone (record values)
+    gives the same bits as in the mixed context."""
+    from morphoshell_amd.meshing import uniform_bbar_info
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    rng = np.random.default_rng(21)
+    m = grf_disc(0.06, seed=21, h=0.02, kappa=0.0)
+    m.thickness = 0.02 * (1.0 + 0.3 * rng.uniform(size=m.num_tris))
+    m.shear_modulus = 1.0 + 0.5 * rng.uniform(size=m.num_tris)
+    m.bbar_info = uniform_bbar_info(m.num_tris, 0.4) + 0.2 * rng.standard_normal((m.num_tris, 3))
+    varied = prepare_shell(m, Settings())
+    uniform = _shell(0.07, seed=22, h=0.02, kappa=0.6)
+    xs = [deformed_state(varied, seed=31), deformed_state(uniform, seed=32)]
+    ends = {}
+    for fused in (0, 1):
+        with _ctx([varied, uniform], precision) as ctx:
+            ctx.set_tuning(node_block=128, fused=fused)
+            for i, st in enumerate(xs):
+                ctx.set_state(i, *st)
+            if fused == 0 and precision == "fp64":
+                ctx.calc_deformation_forces()
+                F = ctx.get_forces(0)
+                Fo = orc.System.from_shell(varied).deformation_forces(xs[0][0])
+                assert np.abs(F - Fo).max() <= 1e-10 * np.abs(Fo).max()
+            ctx.step(20)
+            ends[fused] = [ctx.get_state(i)[0] for i in range(2)]
+    for i in range(2):
+        assert np.array_equal(ends[0][i], ends[1][i]), i
+    with _ctx(uniform, precision) as ctx:
+        ctx.set_tuning(node_block=128, fused=1)
+        ctx.set_state(0, *xs[1])
+        ctx.step(20)
+        assert np.array_equal(ctx.get_state(0)[0], ends[1][1])
+    if precision == "fp64":
+        xo, vo = xs[0][0].copy(), xs[0][1].copy()
+        assert orc.System.from_shell(varied).run_steps(xo, vo, 20) == -1
+        assert np.abs(ends[1][0] - xo).max() <= 1e-11
