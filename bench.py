@@ -70,24 +70,45 @@ def kernel_times(ctx, states, steps: int, strict: bool = True) -> np.ndarray:
     return t
 
 
-def kernel_bytes(T: int, N: int, mixed: bool) -> list[int]:
+def uniform_data(shells) -> bool:
+    """Every shell's thickness, shear modulus and (blend protocol) bbar_info
+    are uniform over its triangles: the kernels then take them from the shell
+    record and do not stream them."""
+    for sh in shells:
+        m = sh.mesh
+        if not (np.all(m.thickness == m.thickness[0]) and np.all(m.shear_modulus == m.shear_modulus[0])):
+            return False
+        if sh.settings.dial_protocol != 3 and not np.all(m.bbar_info == m.bbar_info[0]):
+            return False
+    return True
+
+
+def tri_bytes(mixed: bool, uniform: bool) -> int:
+    """Algorithmic bytes of one triangle record: 6 patch ids (24 B), G and H
+    (192 B FP64 / 96 B FP32), the abar dial info (24 B) and the reference
+    area (8 B / 4 B); unless uniform, also bbar (24 B) and h, mu (16 B / 8 B).
+    288 B FP64 and 180 B mixed in general, 248 B and 148 B with uniform h,
+    mu and bbar."""
+    b = 24 + (96 if mixed else 192) + 24 + (4 if mixed else 8)
+    if not uniform:
+        b += 24 + (8 if mixed else 16)
+    return b
+
+
+def kernel_bytes(T: int, N: int, mixed: bool, uniform: bool) -> list[int]:
     """Algorithmic bytes per launch of the two-launch form.  Triangle kernel:
-    the per-triangle record (6 patch ids 24 B + G, H 192 B FP64 / 96 B FP32 +
-    abar/bbar dial info 48 B + h, mu, area 24 B / 12 B) and each node position
-    once (24 B).  Node kernel: x, v read and written (96 B) and the mass (8 B).
-    The per-slot contributions passed between the two launches are
-    intermediates."""
-    rec = 180 if mixed else 288
-    return [T * rec + N * 24, N * 104]
+    every triangle record once and each node position once (24 B).  Node
+    kernel: x, v read and written (96 B) and the mass (8 B).  The per-slot
+    contributions passed between the two launches are intermediates."""
+    return [T * tri_bytes(mixed, uniform) + N * 24, N * 104]
 
 
-def step_bytes(T: int, N: int, mixed: bool) -> int:
-    """Algorithmic bytes of one step: every triangle record once (288 B FP64 /
-    180 B mixed) and per node x, v read and written plus the mass (104 B).
-    The fused tile kernel does the whole step in one launch, so this is also
-    its per-launch figure; halo triangles that neighbouring tiles read again
-    are not counted."""
-    return T * (180 if mixed else 288) + N * 104
+def step_bytes(T: int, N: int, mixed: bool, uniform: bool) -> int:
+    """Algorithmic bytes of one step: every triangle record once and per node
+    x, v read and written plus the mass (104 B).  The fused tile kernel does
+    the whole step in one launch, so this is also its per-launch figure; halo
+    triangles that neighbouring tiles read again are not counted."""
+    return T * tri_bytes(mixed, uniform) + N * 104
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_grf_disc_1.8M.txt")
@@ -143,6 +164,7 @@ def bench_single(args) -> dict:
     sh = workloads.grf_disc(args.spacing)
     t_setup = time.time() - t0
     T, N = sh.T, sh.N
+    uni = uniform_data([sh])
     log(f"T={T} N={N} mesh + set-up {t_setup:.1f}s")
     x, v = workloads.flat_noisy_state(sh)
     states = [(x, v)]
@@ -163,7 +185,9 @@ def bench_single(args) -> dict:
         cal_ms = np.zeros(1)
         native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
         copy_gbs = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
-        log(f"copy kernel 1 GiB: {cal_ms[0]:.3f} ms = {copy_gbs:.0f} GB/s")
+        native.check(native.lib().ms_calibration_read(1 << 30, 5, native.dptr(cal_ms)))
+        read_gbs = (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
+        log(f"copy kernel 1 GiB: {copy_gbs:.0f} GB/s; read-only stream 1 GiB: {read_gbs:.0f} GB/s")
 
     parity = {}
     if not args.no_parity:
@@ -206,11 +230,11 @@ def bench_single(args) -> dict:
         ms = r["secs"] / args.steps * 1e3
         kt = r["kt"]
         d = {"value": T * args.steps / r["secs"], "ms_per_step": ms,
-             "step_alg_GBs": step_bytes(T, N, mixed) / (ms * 1e-3) / 1e9}
+             "step_alg_GBs": step_bytes(T, N, mixed, uni) / (ms * 1e-3) / 1e9}
         if form == "fused":
             d["kernel_ms"] = {"tile_step": kt[0], "bookkeeping": kt[1]}
         else:
-            kb = kernel_bytes(T, N, mixed)
+            kb = kernel_bytes(T, N, mixed, uni)
             d["kernel_ms"] = {"tri": kt[0], "node": kt[1]}
             d["tri_alg_GBs"] = kb[0] / (kt[0] * 1e-3) / 1e9
         return d
@@ -224,11 +248,11 @@ def bench_single(args) -> dict:
     kt = r0["kt"]
     if main_form == "fused":
         kname = f"tile_step_kernel<{'mixed' if mixed0 else 'fp64'}> (fused forces + gather + dynamics)"
-        kbytes, kms = step_bytes(T, N, mixed0), kt[0]
+        kbytes, kms = step_bytes(T, N, mixed0, uni), kt[0]
     else:
         dom = 0 if kt[0] >= kt[1] else 1
         kname = ["tri_forces_kernel", "node_step_kernel"][dom]
-        kbytes, kms = kernel_bytes(T, N, mixed0)[dom], kt[dom]
+        kbytes, kms = kernel_bytes(T, N, mixed0, uni)[dom], kt[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = measured_traffic(f"tile_step_kernel_{main_prec}") if main_form == "fused" else None
     out = {
@@ -263,6 +287,8 @@ def bench_single(args) -> dict:
             "traffic_detail": traffic,
             "alg_bytes_per_launch": kbytes,
             "alg_bytes_per_triangle": kbytes / T,
+            "tri_record_bytes": tri_bytes(mixed0, uni),
+            "uniform_h_mu_bbar": uni,
             "avg_launch_ms": kms,
         },
         "forms": {f"{p}_{f}": summary(p, f) for (p, f) in res},
@@ -272,6 +298,7 @@ def bench_single(args) -> dict:
     }
     if args.calibrate:
         out["copy_kernel_GBs"] = copy_gbs
+        out["read_stream_GBs"] = read_gbs
     return out
 
 
@@ -286,6 +313,7 @@ def bench_batch(args) -> dict:
     t_setup = time.time() - t0
     T = sum(s.T for s in shells)
     N = sum(s.N for s in shells)
+    uni = uniform_data(shells)
     log(f"batch of {len(shells)} shells: T={T} N={N}, set-up {t_setup:.1f}s")
     states = [workloads.flat_noisy_state(s, seed=i) for i, s in enumerate(shells)]
     with Context(shells, precision="mixed") as ctx:
@@ -308,7 +336,7 @@ def bench_batch(args) -> dict:
         t_relax = time.perf_counter() - t1
     done = sum(1 for r in reps if r["status"] == 2)
     failed = sum(1 for r in reps if r["failed"])
-    kb = kernel_bytes(T, N, True)
+    kb = kernel_bytes(T, N, True, uni)
     return {
         "metric": "triangle force-evals/s (ragged batch)",
         "value": T_live * args.steps / secs,
@@ -324,8 +352,9 @@ def bench_batch(args) -> dict:
         "autotune": tuned,
         "kernel_ms": ({"tile_step": kt[0], "bookkeeping": kt[1]} if tuned.get("fused") else
                       {"tri": kt[0], "node": kt[1]}),
-        "dominant_alg_GBs": (step_bytes(T, N, True) if tuned.get("fused") else kb[0]) / (kt[0] * 1e-3) / 1e9,
-        "step_alg_GBs": step_bytes(T, N, True) / (secs / args.steps) / 1e9,
+        "dominant_alg_GBs": (step_bytes(T, N, True, uni) if tuned.get("fused") else kb[0]) / (kt[0] * 1e-3) / 1e9,
+        "step_alg_GBs": step_bytes(T, N, True, uni) / (secs / args.steps) / 1e9,
+        "tri_record_bytes": tri_bytes(True, uni),
         "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
                   "shells_failed": failed, "shells_per_s": done / t_relax if t_relax > 0 else 0.0,
                   "tolerances": {"force": 1e-6, "speed": 1e-5}},
