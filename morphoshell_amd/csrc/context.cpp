@@ -141,7 +141,8 @@ int build_chunks(ms_ctx* c) {
 void free_tile_records(ms_ctx* c) {
     for (void* p : c->tile_allocs) hipFree(p);
     c->tile_allocs.clear();
-    c->tile.td = TriData{};
+    c->tile.rec64 = nullptr;
+    c->tile.rec32 = nullptr;
     c->dev_bytes -= c->tile_rec_bytes;
     c->tile_rec_bytes = 0;
 }
@@ -157,41 +158,43 @@ void free_tiles(ms_ctx* c) {
     c->tiles_ready = false;
 }
 
-// Tile-ordered copies of the per-triangle streams the mechanics reads
-// (gathered on the device; the patch ids are replaced by tile-local ids).
+// Tile-ordered copies of the per-triangle streams the mechanics reads, in
+// the blocked record layout (see RecSlot; gathered on the device; the patch
+// ids are replaced by tile-local ids).
 int build_tile_records(ms_ctx* c) {
     free_tile_records(c);
     const int T = c->T, Tp = c->tile_T;
-    TriData& o = c->tile.td;
-    o.T = Tp;
-    o.N = c->N;
-    auto copy = [&](const void* in, int elem_bytes, int ncomp, const void** out) -> hipError_t {
-        *out = nullptr;
-        if (!in) return hipSuccess;
-        void* p = nullptr;
-        const size_t nb = (size_t)elem_bytes * ncomp * (size_t)std::max(1, Tp);
-        hipError_t e = hipMalloc(&p, nb);
+    const size_t vals = (size_t)((Tp + 63) / 64) * REC_SLOTS * 64;
+    auto alloc = [&](size_t bytes, void** out) -> hipError_t {
+        hipError_t e = hipMalloc(out, bytes);
         if (e != hipSuccess) return e;
-        c->tile_allocs.push_back(p);
-        c->tile_rec_bytes += (int64_t)nb;
-        c->dev_bytes += (int64_t)nb;
-        launch_tile_gather(c->stream, in, p, elem_bytes, ncomp, T, Tp, c->tile.tris);
-        *out = p;
-        return hipGetLastError();
+        c->tile_allocs.push_back(*out);
+        c->tile_rec_bytes += (int64_t)bytes;
+        c->dev_bytes += (int64_t)bytes;
+        return hipSuccess;
     };
-    HIPC(c, copy(c->G, 8, 6, (const void**)&o.G));
-    HIPC(c, copy(c->H, 8, 18, (const void**)&o.H));
-    HIPC(c, copy(c->G32, 4, 6, (const void**)&o.G32));
-    HIPC(c, copy(c->H32, 4, 18, (const void**)&o.H32));
-    HIPC(c, copy(c->ainfo, 8, c->acols, (const void**)&o.ainfo));
-    HIPC(c, copy(c->binfo, 8, 3, (const void**)&o.binfo));
-    HIPC(c, copy(c->a0, 8, 3, (const void**)&o.a0));
-    HIPC(c, copy(c->b0, 8, 3, (const void**)&o.b0));
-    HIPC(c, copy(c->h, 8, 1, (const void**)&o.h));
-    HIPC(c, copy(c->mu, 8, 1, (const void**)&o.mu));
-    HIPC(c, copy(c->area, 8, 1, (const void**)&o.area));
-    HIPC(c, copy(c->hmuA32, 4, 3, (const void**)&o.hmuA32));
+    void *r64 = nullptr, *r32 = nullptr;
+    HIPC(c, alloc(vals * sizeof(double), &r64));
+    HIPC(c, alloc(vals * sizeof(float), &r32));
+    auto gather = [&](const void* in, void* out, int elem_bytes, int ncomp, int slot) {
+        if (in) launch_tile_gather(c->stream, in, out, elem_bytes, ncomp, T, Tp, slot, c->tile.tris);
+    };
+    gather(c->G, r64, 8, 6, RS_G);
+    gather(c->H, r64, 8, 18, RS_H);
+    gather(c->ainfo, r64, 8, c->acols, RS_AI);
+    gather(c->binfo, r64, 8, 3, RS_BI);
+    gather(c->h, r64, 8, 1, RS_THICK);
+    gather(c->mu, r64, 8, 1, RS_MU);
+    gather(c->area, r64, 8, 1, RS_AREA);
+    gather(c->a0, r64, 8, 3, RS_A0);
+    gather(c->b0, r64, 8, 3, RS_B0);
+    gather(c->G32, r32, 4, 6, RS_G);
+    gather(c->H32, r32, 4, 18, RS_H);
+    gather(c->hmuA32, r32, 4, 3, RS_HMUA);
+    HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
+    c->tile.rec64 = (const double*)r64;
+    c->tile.rec32 = (const float*)r32;
     return MS_OK;
 }
 
@@ -203,7 +206,7 @@ int build_tiles(ms_ctx* c) {
     free_tiles(c);
     const int T = c->T, N = c->N;
     const int B = c->tune.node_block;
-    const int split = B;
+    const int split = B;  // a multiple of 64: each wave's entries are one record block
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
     std::vector<Chunk> nc(c->n_node_chunks);
@@ -223,7 +226,7 @@ int build_tiles(ms_ctx* c) {
         tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
         max_passes = std::max(max_passes, (int)((tl.size() + split - 1) / split));
     }
-    std::vector<int> toff(nc.size() + 1, 0), tris, hoff(nc.size() + 1, 0), halo;
+    std::vector<int> toff(nc.size() + 1, 0), tend(nc.size(), 0), tris, hoff(nc.size() + 1, 0), halo;
     std::vector<unsigned short> lpid, dst, poff(nc.size() * (size_t)max_passes * stride, 0);
     int max_entries = 0, max_nodes = 0;
     std::vector<int> hl;
@@ -254,6 +257,16 @@ int build_tiles(ms_ctx* c) {
                 cnt[jj / split][i] += 1;
             }
         unsigned short* pbase = &poff[ci * (size_t)max_passes * stride];
+        // the tile's entries start at a multiple of 64 (padding entries: no
+        // triangle, no own slot)
+        while (tris.size() % 64) {
+            tris.push_back(-1);
+            for (int s = 0; s < 6; ++s) {
+                lpid.push_back(0);
+                dst.push_back(0xFFFF);
+            }
+        }
+        toff[ci] = (int)tris.size();
         for (int p = 0; p < npass; ++p) {
             unsigned short* po = pbase + (size_t)p * stride;
             for (int i = 0; i < nn; ++i) {
@@ -290,10 +303,13 @@ int build_tiles(ms_ctx* c) {
                 dst.push_back(q);
             }
         }
-        toff[ci + 1] = (int)tris.size();
+        tend[ci] = (int)tris.size();
         halo.insert(halo.end(), hl.begin(), hl.end());
         hoff[ci + 1] = (int)halo.size();
     }
+    toff[nc.size()] = (int)tris.size();
+    if (tris.size() > (size_t)INT_MAX / 6)
+        return fail(c, MS_ERR_ARG, "fused step: too many tile entries for 32-bit indexing");
     TileData& td = c->tile;
     td.poff_stride = stride;
     td.pass_split = split;
@@ -316,9 +332,10 @@ int build_tiles(ms_ctx* c) {
         *dptr = static_cast<typename std::remove_pointer<decltype(dptr)>::type>(p);
         return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice);
     };
-    int *d_toff, *d_tris, *d_hoff, *d_halo;
+    int *d_toff, *d_tend, *d_tris, *d_hoff, *d_halo;
     unsigned short *d_lpid, *d_dst, *d_poff;
     HIPC(c, upload(toff, &d_toff));
+    HIPC(c, upload(tend, &d_tend));
     HIPC(c, upload(tris, &d_tris));
     HIPC(c, upload(hoff, &d_hoff));
     HIPC(c, upload(halo, &d_halo));
@@ -326,6 +343,7 @@ int build_tiles(ms_ctx* c) {
     HIPC(c, upload(dst, &d_dst));
     HIPC(c, upload(poff, &d_poff));
     td.tri_off = d_toff;
+    td.tri_end = d_tend;
     td.tris = d_tris;
     td.halo_off = d_hoff;
     td.halo = d_halo;

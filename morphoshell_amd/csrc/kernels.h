@@ -106,11 +106,34 @@ void launch_copy(hipStream_t s, const void* src, void* dst, long long nbytes);
 void launch_read(hipStream_t s, const void* src, float* out, long long nbytes);
 long long read_out_floats();
 
+// Tile-ordered triangle records, blocked by 64 entries: entry j lies in block
+// j / 64 at lane j % 64, and a block holds REC_SLOTS component slots of 64
+// values each, so the components of 64 consecutive entries are one
+// contiguous run and one wave's loads address them from a single base with
+// constant offsets.  Slots of the FP64 array rec64 and of the FP32 array
+// rec32 (same blocking):
+enum RecSlot : int {
+    RS_G = 0,       // G (6)          rec64, rec32
+    RS_H = 6,       // H (18)         rec64, rec32
+    RS_AI = 24,     // dial info (up to 5)    rec64
+    RS_BI = 29,     // bbar info (3)          rec64
+    RS_THICK = 32,  // h                      rec64
+    RS_MU = 33,     // mu                     rec64
+    RS_AREA = 34,   // ref area               rec64
+    RS_A0 = 35,     // ansatz a (3)           rec64
+    RS_B0 = 38,     // ansatz b (3)           rec64
+    RS_HMUA = 24,   // (h, mu, area) (3)      rec32
+    REC_SLOTS = 41,
+};
+
 // Fused owner-computes step.  Tile c = node chunk c (at most B = block size
-// nodes).  Entries j in [tri_off[c], tri_off[c+1]) are the triangles
-// (ascending) whose patches touch the tile's nodes; td holds a copy of their
-// per-triangle records at index j (halo triangles appear in several tiles),
-// so every tile streams its records contiguously.  The tile's nodes and the
+// nodes).  Entries j in [tri_off[c], tri_end[c]) are the triangles
+// (ascending) whose patches touch the tile's nodes; rec64/rec32 hold a copy
+// of their per-triangle records at entry j (halo triangles appear in several
+// tiles), so every tile streams its records contiguously.  tri_off[c] is a
+// multiple of 64 (the entry lists are padded between tiles), so with passes
+// of a multiple of 64 entries each wave's entries form one record block.
+// The tile's nodes and the
 // halo nodes [halo_off[c], halo_off[c+1]) of its triangles are staged in LDS
 // (tile-local ids: own node n -> n - first, halo node k -> own count + k);
 // lpid[6j + slot] is the tile-local id of a patch node.  The triangles run
@@ -119,8 +142,9 @@ long long read_out_floats();
 // lies outside the tile, and poff[(c * max_passes + pass) * poff_stride + i]
 // is the first pass-local position of own node i's entries.
 struct TileData {
-    const int* tri_off;            // nchunks + 1
-    const int* tris;               // triangle of entry j
+    const int* tri_off;            // nchunks + 1 (first entry of each tile, multiple of 64)
+    const int* tri_end;            // nchunks (one past the tile's last entry)
+    const int* tris;               // triangle of entry j (-1 for padding)
     const unsigned short* lpid;    // 6 per entry
     const unsigned short* dst;     // 6 per entry
     const int* halo_off;           // nchunks + 1
@@ -131,10 +155,12 @@ struct TileData {
     int max_passes;
     int max_nodes;                 // max over tiles of own + halo nodes
     int max_entries;               // max over tiles and passes of pass entries
-    TriData td;                    // tile-ordered records, td.T = tri_off[nchunks]
+    const double* rec64;           // REC_SLOTS * 64 doubles per block of 64 entries
+    const float* rec32;            // REC_SLOTS * 64 floats per block
 };
-// out[k*Tp + j] = in[k*T + tris[j]] for k < ncomp, j < Tp (elem_bytes 4 or 8).
-void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp,
+// Record gather: component k < ncomp of stream `in` (SoA, [k*T + t]) into
+// slot `slot + k` of every entry j < Tp with tris[j] >= 0 (elem_bytes 4 or 8).
+void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp, int slot,
                         const int* tris);
 int tile_lds_bytes(const TileData& tl, bool mixed);
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.

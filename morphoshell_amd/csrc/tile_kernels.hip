@@ -83,10 +83,13 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
 // otherwise cost a workgroup per CU), then contributions (3 CT per pass
 // entry).
 // Waves per SIMD asked of the compiler (their LDS tiles allow them): without
-// the LCE protocol 3 in FP64 (168 registers, no spill) and 4 in mixed mode
-// (a few bytes spilled); with it, 3 in mixed mode and what fits in FP64.
+// the LCE protocol 2 in FP64 and 4 in mixed mode; with it, 3 in mixed mode
+// and what fits in FP64.  Each triangle's record is loaded whole before its
+// first use; at 2 waves the FP64 kernel keeps every wave's record in flight
+// without spilling, and measured faster than at 3 waves (168 registers, where
+// the loads had to wait for registers).
 template <bool MIXED, bool CHECK, bool LCE, bool UNI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 1) : (MIXED ? 4 : 3))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 1) : (MIXED ? 4 : 2))))
 void tile_step_kernel(TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
     extern __shared__ double smem[];
@@ -114,7 +117,7 @@ void tile_step_kernel(TileKArgs k) {
             lpos[3 * i + 2] = ld(X + 2 * (size_t)N, g);
         }
         __syncthreads();
-        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_off[ci + 1]);
+        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_end[ci]);
         const int split = k.tl.pass_split;
         const int npass = (j1 - j0 + split - 1) / split;
         const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * k.tl.poff_stride;
@@ -134,11 +137,30 @@ void tile_step_kernel(TileKArgs k) {
                 const unsigned w0 = ld(lw, 3 * t), w1 = ld(lw, 3 * t + 1), w2 = ld(lw, 3 * t + 2);
                 const unsigned lp[6] = {w0 & 0xFFFFu, w0 >> 16, w1 & 0xFFFFu, w1 >> 16, w2 & 0xFFFFu, w2 >> 16};
                 const TilePos pos{lpos, lp};
+                // this wave's record block (its 64 entries start at a multiple of 64)
+                const size_t blk = (size_t)uni(j >> 6) * (REC_SLOTS * 64);
+                const double* r64 = k.tl.rec64 + blk;
+                const float* r32 = k.tl.rec32 + blk;
+                TriData rv{};
+                rv.T = 64;
+                rv.G = r64 + RS_G * 64;
+                rv.H = r64 + RS_H * 64;
+                rv.ainfo = r64 + RS_AI * 64;
+                rv.binfo = r64 + RS_BI * 64;
+                rv.h = r64 + RS_THICK * 64;
+                rv.mu = r64 + RS_MU * 64;
+                rv.area = r64 + RS_AREA * 64;
+                rv.a0 = r64 + RS_A0 * 64;
+                rv.b0 = r64 + RS_B0 * 64;
+                rv.G32 = r32 + RS_G * 64;
+                rv.H32 = r32 + RS_H * 64;
+                rv.hmuA32 = r32 + RS_HMUA * 64;
+                const unsigned lane = (unsigned)j & 63u;
                 CT out[18];
                 if (MIXED)
-                    tri_forces_mixed<LCE, UNI>(k.tl.td, tp, t, pos, reinterpret_cast<float*>(out));
+                    tri_forces_mixed<LCE, UNI, 64>(rv, tp, lane, pos, reinterpret_cast<float*>(out));
                 else
-                    tri_forces_fp64<LCE, UNI, false>(k.tl.td, tp, t, pos, reinterpret_cast<double*>(out));
+                    tri_forces_fp64<LCE, UNI, true, 64>(rv, tp, lane, pos, reinterpret_cast<double*>(out));
                 const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
                 const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
                 const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
@@ -233,28 +255,30 @@ void tile_step_kernel(TileKArgs k) {
 
 template <typename E>
 __global__ __launch_bounds__(256) void tile_gather_kernel(const E* __restrict__ in, E* __restrict__ out, int ncomp,
-                                                          int T, int Tp, const int* __restrict__ tris) {
+                                                          int T, int Tp, int slot, const int* __restrict__ tris) {
     const long long total = (long long)ncomp * Tp;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int kc = (int)(i / Tp), j = (int)(i - (long long)kc * Tp);
-        out[i] = in[(size_t)kc * T + tris[j]];
+        const int t = tris[j];
+        if (t < 0) continue;  // padding between tiles
+        out[((size_t)(j >> 6) * REC_SLOTS + slot + kc) * 64 + (j & 63)] = in[(size_t)kc * T + t];
     }
 }
 
 }  // namespace
 
-void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp,
+void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp, int slot,
                         const int* tris) {
     const long long total = (long long)ncomp * Tp;
     if (total <= 0) return;
     const int grid = (int)std::min<long long>(8192, (total + 255) / 256);
     if (elem_bytes == 8)
         hipLaunchKernelGGL(tile_gather_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)in, (double*)out,
-                           ncomp, T, Tp, tris);
+                           ncomp, T, Tp, slot, tris);
     else
         hipLaunchKernelGGL(tile_gather_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)in, (float*)out,
-                           ncomp, T, Tp, tris);
+                           ncomp, T, Tp, slot, tris);
 }
 
 int tile_lds_bytes(const TileData& tl, bool mixed) {

@@ -102,9 +102,11 @@ struct DialRec {
     double ai[5], bi[3], A0[3], B0[3];
 };
 
-template <bool LCE, bool UNI>
+// CS: component stride of the record streams when known at compile time (the
+// tile launch's blocked records, 64), else 0 for td.T.
+template <bool LCE, bool UNI, int CS = 0>
 __device__ __forceinline__ void load_dial(const TriData& td, const TriParams& s, unsigned t, DialRec& d) {
-    const size_t T = (size_t)td.T;
+    const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
     for (int k = 0; k < 3; ++k) d.ai[k] = ld(td.ainfo + k * T, t);
     if (LCE && s.protocol == 3) {
@@ -163,11 +165,11 @@ __device__ __forceinline__ void dial_targets(const DialRec& d, const TriParams& 
 }
 
 // Programmed abar, bbar of triangle t at the shell's dial factor.
-template <bool LCE = true, bool UNI = false>
+template <bool LCE = true, bool UNI = false, int CS = 0>
 __device__ __forceinline__ void targets(const TriData& td, const TriParams& s, unsigned t, double ab[3],
                                         double bb[3]) {
     DialRec d;
-    load_dial<LCE, UNI>(td, s, t, d);
+    load_dial<LCE, UNI, CS>(td, s, t, d);
     dial_targets<LCE>(d, s, ab, bb);
 }
 
@@ -233,14 +235,14 @@ struct TilePos {
 
 // FP64 force contributions of triangle t (out[3*slot + c]); pos fetches its
 // six patch nodes.  EARLY issues the whole record (patch coefficients, dial
-// data, h, mu, area) before any use, one memory round trip per triangle:
-// faster in the triangle launch; the tile launch, held at 168 registers,
-// keeps more in flight with the loads at their uses.  Same arithmetic.
-template <bool LCE, bool UNI, bool EARLY, class Pos>
+// data, h, mu, area) before any use, one memory round trip per triangle;
+// without it the loads sit at their uses (fewer registers live).  Same
+// arithmetic either way.
+template <bool LCE, bool UNI, bool EARLY, int CS, class Pos>
 __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                 double out[18]) {
 #pragma clang fp contract(on)
-    const size_t T = (size_t)td.T;
+    const size_t T = CS ? (size_t)CS : (size_t)td.T;
     double gc[6], hc[18], h, mu, A;
     DialRec dr;
     if constexpr (EARLY) {
@@ -251,7 +253,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
         h = UNI ? sp.h : ld(td.h, t);
         mu = UNI ? sp.mu : ld(td.mu, t);
         A = ld(td.area, t);
-        load_dial<LCE, UNI>(td, sp, t, dr);
+        load_dial<LCE, UNI, CS>(td, sp, t, dr);
     }
     const double nu = sp.nu;
     double Px[6], Py[6], Pz[6];
@@ -300,7 +302,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
     if constexpr (EARLY) {
         dial_targets<LCE>(dr, sp, ab, bb);
     } else {
-        targets<LCE, UNI>(td, sp, t, ab, bb);
+        targets<LCE, UNI, CS>(td, sp, t, ab, bb);
         h = UNI ? sp.h : ld(td.h, t);
         mu = UNI ? sp.mu : ld(td.mu, t);
         A = ld(td.area, t);
@@ -356,11 +358,11 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-template <bool LCE, bool UNI, class Pos>
+template <bool LCE, bool UNI, int CS, class Pos>
 __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
                                                  float out[18]) {
 #pragma clang fp contract(on)
-    const size_t T = (size_t)td.T;
+    const size_t T = CS ? (size_t)CS : (size_t)td.T;
     // Edge vectors, the surface derivatives and the strains a - abar, b - bbar
     // are formed in FP64 (the FP32 patch-fit coefficients are exact in
     // double).  Rounding the tangent vectors to FP32 before forming the
@@ -381,7 +383,7 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
     const float mu = UNI ? (float)sp.mu : ld(td.hmuA32 + T, t);
     const float A = ld(td.hmuA32 + 2 * T, t);
     DialRec dr;
-    load_dial<LCE, UNI>(td, sp, t, dr);
+    load_dial<LCE, UNI, CS>(td, sp, t, dr);
     double x0, y0, z0;
     pos(0, x0, y0, z0);
     // one edge vector at a time into the first and second derivatives
