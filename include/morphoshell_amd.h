@@ -327,11 +327,14 @@ typedef struct ms_tuning {
     int32_t node_grid_per_cu;
     int32_t graph_steps;      /* steps captured per HIP graph */
     int32_t fused;            /* 1 = one owner-computes tile launch per step (a
-                                 tile is a node chunk; its workgroup recomputes
+                                 tile is tile_chunks node chunks; its workgroup recomputes
                                  the halo triangles and gathers from LDS);
                                  0 = triangle launch + node launch; -1 keeps.
                                  All give bit-identical states. */
-    int32_t reserved[2];
+    int32_t tile_chunks;      /* node chunks per tile of the fused step: 1 or 2
+                                 (larger tiles recompute fewer halo triangles);
+                                 0 keeps */
+    int32_t reserved;
 } ms_tuning;
 int ms_set_tuning(ms_ctx* ctx, const ms_tuning* t);
 int ms_get_tuning(const ms_ctx* ctx, ms_tuning* t);

@@ -26,20 +26,24 @@ DEFAULT_SPACE = {
     "node_block": (128, 256),
     "grid_per_cu": (0, 4, 8),
     "graph_steps": (16,),
-    "fused_node_block": (128, 256),  # tile sizes of the fused step
+    "fused_node_block": (128, 256),  # block sizes of the fused step
+    "fused_tile_chunks": (1, 2),      # node chunks per tile of the fused step
 }
 
 
 def _candidates(sp: dict):
     for tb, nb, g, gs in itertools.product(sp["tri_block"], sp["node_block"], sp["grid_per_cu"], sp["graph_steps"]):
         yield {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs, "fused": 0}
-    for nb, g, gs in itertools.product(sp.get("fused_node_block", ()), sp["grid_per_cu"], sp["graph_steps"]):
-        yield {"tri_block": 256, "node_block": nb, "grid_per_cu": g, "graph_steps": gs, "fused": 1}
+    for nb, tc, g, gs in itertools.product(sp.get("fused_node_block", ()), sp.get("fused_tile_chunks", (1,)),
+                                           sp["grid_per_cu"], sp["graph_steps"]):
+        yield {"tri_block": 256, "node_block": nb, "tile_chunks": tc, "grid_per_cu": g, "graph_steps": gs,
+               "fused": 1}
 
 
 def _apply(ctx: Context, c: dict) -> None:
     ctx.set_tuning(tri_block=c["tri_block"], node_block=c["node_block"], tri_grid_per_cu=c["grid_per_cu"],
-                   node_grid_per_cu=c["grid_per_cu"], graph_steps=c["graph_steps"], fused=c.get("fused", 0))
+                   node_grid_per_cu=c["grid_per_cu"], graph_steps=c["graph_steps"], fused=c.get("fused", 0),
+                   tile_chunks=c.get("tile_chunks", 1))
 
 
 def _cache_path() -> str:
@@ -55,7 +59,7 @@ def signature(ctx: Context) -> str:
     S = len(ctx.shells)
     b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
     prec = "mixed" if ctx.precision == 1 else "fp64"
-    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-v2"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-v3"
 
 
 def _load() -> dict:

@@ -206,7 +206,12 @@ int build_tiles(ms_ctx* c) {
     free_tiles(c);
     const int T = c->T, N = c->N;
     const int B = c->tune.node_block;
-    const int split = B;  // a multiple of 64: each wave's entries are one record block
+    // Passes of 2B triangles in FP64 (two workgroups per CU leave the LDS
+    // for it, and half the barriers measured 3% faster), B in mixed mode
+    // (four workgroups per CU).  Chosen when the tiles are built; a later
+    // precision switch keeps them (correct either way).  A multiple of 64:
+    // each wave's entries are one record block.
+    const int split = c->precision == MS_MIXED ? B : 2 * B;
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
     std::vector<Chunk> nc(c->n_node_chunks);
@@ -215,25 +220,36 @@ int build_tiles(ms_ctx* c) {
     ent.resize((size_t)off[N]);
     HIPC(c, hipMemcpy(ent.data(), c->csr_ent, ent.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPC(c, hipMemcpy(nc.data(), c->d_node_chunks, nc.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
-    const int stride = B + 1;
+    // tiles: groups of up to tile_chunks consecutive chunks of one shell
+    const int npt = std::max(1, std::min(MAX_TILE_CHUNKS, c->tune.tile_chunks));
+    std::vector<int4> tiles;
+    for (int s = 0; s < c->num_shells; ++s) {
+        const ShellDev& S = c->hs[s];
+        for (int k = 0; k < S.node_chunk_count; k += npt) {
+            const int first = S.node_chunk_begin + k;
+            const int last = S.node_chunk_begin + std::min(k + npt, S.node_chunk_count) - 1;
+            tiles.push_back(make_int4(nc[first].x, nc[last].y, s, first));
+        }
+    }
+    const int stride = npt * B + 1;
     // pass count of the largest tile first (it sizes the offset table)
     int max_passes = 1;
-    std::vector<std::vector<int>> tile_tris(nc.size());
-    for (size_t ci = 0; ci < nc.size(); ++ci) {
+    std::vector<std::vector<int>> tile_tris(tiles.size());
+    for (size_t ci = 0; ci < tiles.size(); ++ci) {
         std::vector<int>& tl = tile_tris[ci];
-        for (long long e = off[nc[ci].x]; e < off[nc[ci].y]; ++e) tl.push_back(ent[e] / 6);
+        for (long long e = off[tiles[ci].x]; e < off[tiles[ci].y]; ++e) tl.push_back(ent[e] / 6);
         std::sort(tl.begin(), tl.end());
         tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
         max_passes = std::max(max_passes, (int)((tl.size() + split - 1) / split));
     }
-    std::vector<int> toff(nc.size() + 1, 0), tend(nc.size(), 0), tris, hoff(nc.size() + 1, 0), halo;
-    std::vector<unsigned short> lpid, dst, poff(nc.size() * (size_t)max_passes * stride, 0);
+    std::vector<int> toff(tiles.size() + 1, 0), tend(tiles.size(), 0), tris, hoff(tiles.size() + 1, 0), halo;
+    std::vector<unsigned short> lpid, dst, poff(tiles.size() * (size_t)max_passes * stride, 0);
     int max_entries = 0, max_nodes = 0;
     std::vector<int> hl;
     std::vector<std::vector<int>> cnt;
-    for (size_t ci = 0; ci < nc.size(); ++ci) {
-        const int n0 = nc[ci].x, n1 = nc[ci].y, nn = n1 - n0;
-        if (nn > B) return fail(c, MS_ERR_ARG, "fused step: a node chunk is larger than the block");
+    for (size_t ci = 0; ci < tiles.size(); ++ci) {
+        const int n0 = tiles[ci].x, n1 = tiles[ci].y, nn = n1 - n0;
+        if (nn > npt * B) return fail(c, MS_ERR_ARG, "fused step: a tile is larger than its chunks");
         const std::vector<int>& tl = tile_tris[ci];
         const int npass = (int)((tl.size() + split - 1) / split);
         hl.clear();
@@ -307,7 +323,7 @@ int build_tiles(ms_ctx* c) {
         halo.insert(halo.end(), hl.begin(), hl.end());
         hoff[ci + 1] = (int)halo.size();
     }
-    toff[nc.size()] = (int)tris.size();
+    toff[tiles.size()] = (int)tris.size();
     if (tris.size() > (size_t)INT_MAX / 6)
         return fail(c, MS_ERR_ARG, "fused step: too many tile entries for 32-bit indexing");
     TileData& td = c->tile;
@@ -333,7 +349,9 @@ int build_tiles(ms_ctx* c) {
         return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice);
     };
     int *d_toff, *d_tend, *d_tris, *d_hoff, *d_halo;
+    int4* d_tiles;
     unsigned short *d_lpid, *d_dst, *d_poff;
+    HIPC(c, upload(tiles, &d_tiles));
     HIPC(c, upload(toff, &d_toff));
     HIPC(c, upload(tend, &d_tend));
     HIPC(c, upload(tris, &d_tris));
@@ -342,6 +360,8 @@ int build_tiles(ms_ctx* c) {
     HIPC(c, upload(lpid, &d_lpid));
     HIPC(c, upload(dst, &d_dst));
     HIPC(c, upload(poff, &d_poff));
+    td.tiles = d_tiles;
+    td.ntiles = (int)tiles.size();
     td.tri_off = d_toff;
     td.tri_end = d_tend;
     td.tris = d_tris;
@@ -443,8 +463,8 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
-        launch_tile_step(node_launch(c), nd, c->b, tile_data(c), c->d_node_chunks, c->n_node_chunks, c->d_shells,
-                         mixed, has_lce(c), uni_data(c), fl);
+        const Launch tl{grid_for(c, c->tile.ntiles, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
+        launch_tile_step(tl, nd, c->b, tile_data(c), c->d_shells, mixed, has_lce(c), uni_data(c), fl);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                                 c->d_clock, fl);
@@ -536,6 +556,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     c->tune.node_grid_per_cu = 0;
     c->tune.graph_steps = 16;
     c->tune.fused = 0;
+    c->tune.tile_chunks = 1;
     c->acols = acols;
     int dev = 0;
     hipGetDevice(&dev);
@@ -1287,7 +1308,13 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
         n.fused = t->fused;
     else if (t->fused != -1)
         return fail(c, MS_ERR_ARG, "fused must be 0, 1 or -1");
+    if (t->tile_chunks) {
+        if (t->tile_chunks < 1 || t->tile_chunks > MAX_TILE_CHUNKS)
+            return fail(c, MS_ERR_ARG, fmt("tile_chunks must be 1 to %d (0 keeps)", MAX_TILE_CHUNKS));
+        n.tile_chunks = t->tile_chunks;
+    }
     const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
+    const bool retile = rechunk || n.tile_chunks != c->tune.tile_chunks;
     if (int r = pull_shells(c)) return r;
     const ms_tuning old = c->tune;
     c->tune = n;
@@ -1296,6 +1323,7 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
         if (int r = build_chunks(c)) return r;
         if (int r = push_shells(c)) return r;
     }
+    if (retile) c->tiles_ready = false;
     if (n.fused == 1 && !c->tiles_ready) {
         if (int r = build_tiles(c)) {
             c->tune.fused = old.fused;  // keep a working configuration

@@ -126,8 +126,10 @@ enum RecSlot : int {
     REC_SLOTS = 41,
 };
 
-// Fused owner-computes step.  Tile c = node chunk c (at most B = block size
-// nodes).  Entries j in [tri_off[c], tri_end[c]) are the triangles
+// Fused owner-computes step.  Tile c = tiles[c] = {first node, end node,
+// shell, first chunk}: up to MAX_TILE_CHUNKS consecutive node chunks of one
+// shell (B = block size nodes each; node i of the tile lies in chunk
+// first + i / B).  Entries j in [tri_off[c], tri_end[c]) are the triangles
 // (ascending) whose patches touch the tile's nodes; rec64/rec32 hold a copy
 // of their per-triangle records at entry j (halo triangles appear in several
 // tiles), so every tile streams its records contiguously.  tri_off[c] is a
@@ -141,17 +143,20 @@ enum RecSlot : int {
 // position of the patch-incidence entry (j, slot), or 0xFFFF when that node
 // lies outside the tile, and poff[(c * max_passes + pass) * poff_stride + i]
 // is the first pass-local position of own node i's entries.
+constexpr int MAX_TILE_CHUNKS = 2;
 struct TileData {
-    const int* tri_off;            // nchunks + 1 (first entry of each tile, multiple of 64)
-    const int* tri_end;            // nchunks (one past the tile's last entry)
+    const int4* tiles;             // ntiles
+    int ntiles;
+    const int* tri_off;            // ntiles + 1 (first entry of each tile, multiple of 64)
+    const int* tri_end;            // ntiles (one past the tile's last entry)
     const int* tris;               // triangle of entry j (-1 for padding)
     const unsigned short* lpid;    // 6 per entry
     const unsigned short* dst;     // 6 per entry
-    const int* halo_off;           // nchunks + 1
+    const int* halo_off;           // ntiles + 1
     const int* halo;               // global ids of halo nodes
     const unsigned short* poff;    // max_passes * poff_stride per tile
-    int poff_stride;               // B + 1
-    int pass_split;                // entries per pass (B)
+    int poff_stride;               // tile nodes + 1
+    int pass_split;                // entries per pass (B or 2B)
     int max_passes;
     int max_nodes;                 // max over tiles of own + halo nodes
     int max_entries;               // max over tiles and passes of pass entries
@@ -171,8 +176,8 @@ hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // kernel variant that evaluates it).  uni_data: every shell has uniform
 // thickness, shear modulus and (blend protocol) bbar_info (ShellDev h_u, mu_u,
 // b_u), so the variant that reads them from the shell record runs.
-void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, bool lce, bool uni_data, const StepFlags& fl);
+void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, ShellDev* shells,
+                      bool mixed, bool lce, bool uni_data, const StepFlags& fl);
 // The per-step bookkeeping launch of launch_node_step (one block).
 void launch_step_bookkeeping(hipStream_t s, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                              ShellDev* shells, int num_shells, ClockDev* clock, const StepFlags& fl);

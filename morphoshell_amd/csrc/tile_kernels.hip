@@ -30,8 +30,6 @@ struct TileKArgs {
     NodeData nd;
     Buffers b;
     TileData tl;
-    const Chunk* chunks;
-    int nchunks;
     ShellDev* shells;
     StepFlags fl;
 };
@@ -95,12 +93,15 @@ void tile_step_kernel(TileKArgs k) {
     extern __shared__ double smem[];
     const int N = k.nd.N;
     const int B = blockDim.x;
+    const int stride = k.tl.poff_stride;
     double* lpos = smem;
     double* lsum = smem + 3 * (size_t)k.tl.max_nodes;
-    CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)B : lsum);
-    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
-        const int ci = xcd_remap(vi, k.nchunks);
-        const Chunk ch = make_int4(uni(k.chunks[ci].x), uni(k.chunks[ci].y), uni(k.chunks[ci].z), 0);
+    CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)stride : lsum);
+    for (int vi = blockIdx.x; vi < k.tl.ntiles; vi += gridDim.x) {
+        const int ci = xcd_remap(vi, k.tl.ntiles);
+        // nodes [ch.x, ch.y) of shell ch.z; chunk ch.w + r holds nodes from ch.x + r * B
+        const int4 ch = make_int4(uni(k.tl.tiles[ci].x), uni(k.tl.tiles[ci].y), uni(k.tl.tiles[ci].z),
+                                  uni(k.tl.tiles[ci].w));
         const ShellDev& sh = k.shells[ch.z];
         if (uni(sh.halted)) continue;  // block-uniform
         const bool do_check = CHECK || uni(sh.check_due);
@@ -120,8 +121,10 @@ void tile_step_kernel(TileKArgs k) {
         const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_end[ci]);
         const int split = k.tl.pass_split;
         const int npass = (j1 - j0 + split - 1) / split;
-        const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * k.tl.poff_stride;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * stride;
+        double fx[MAX_TILE_CHUNKS], fy[MAX_TILE_CHUNKS], fz[MAX_TILE_CHUNKS];
+#pragma unroll
+        for (int r = 0; r < MAX_TILE_CHUNKS; ++r) fx[r] = fy[r] = fz[r] = 0.0;
 #pragma unroll 1
         for (int pass = 0; pass < npass; ++pass) {
             // 2. this pass's triangles into LDS
@@ -176,77 +179,94 @@ void tile_step_kernel(TileKArgs k) {
             }
             __syncthreads();
             // 3. each node adds this pass's entries, in CSR order
-            if ((int)threadIdx.x < nn) {
-                const unsigned short* po = poff + pass * k.tl.poff_stride;
+            const unsigned short* po = poff + pass * stride;
+#pragma unroll
+            for (int r = 0; r < MAX_TILE_CHUNKS; ++r) {
+                const int i = (int)threadIdx.x + r * B;
+                if (i >= nn) continue;
                 if (MIXED) {
-                    double* ls = lsum + 3 * threadIdx.x;
+                    double* ls = lsum + 3 * i;
                     double sx = 0.0, sy = 0.0, sz = 0.0;
                     if (pass > 0) {
                         sx = ls[0];
                         sy = ls[1];
                         sz = ls[2];
                     }
-                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
-                                     (long long)po[threadIdx.x + 1], sx, sy, sz);
+                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[i], (long long)po[i + 1], sx,
+                                     sy, sz);
                     ls[0] = sx;
                     ls[1] = sy;
                     ls[2] = sz;
                 } else {
-                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[threadIdx.x],
-                                     (long long)po[threadIdx.x + 1], fx, fy, fz);
+                    ordered_sum3_acc(lcon, [](long long e) { return e; }, (long long)po[i], (long long)po[i + 1],
+                                     fx[r], fy[r], fz[r]);
                 }
             }
             __syncthreads();  // the next pass (or tile) reuses the LDS
         }
-        if (MIXED && (int)threadIdx.x < nn && npass > 0) {
-            fx = lsum[3 * threadIdx.x];
-            fy = lsum[3 * threadIdx.x + 1];
-            fz = lsum[3 * threadIdx.x + 2];
-        }
-        // the node's update
-        const int n = ch.x + threadIdx.x;
-        double fnorm = -1.0, vnorm = -1.0;
-        int fidx = INT_MAX, vidx = INT_MAX;
-        if (n < ch.y) {
-            const double* V = k.b.v[par];
-            double* Xo = k.b.x[par ^ 1];
-            double* Vo = k.b.v[par ^ 1];
-            double vx = V[n], vy = V[N + n], vz = V[2 * N + n];
-            const double m = k.nd.mass[n];
-            if (!sh.gd) {
-                fx = damped(fx, sh.eta, m, vx, sh.damp_den);
-                fy = damped(fy, sh.eta, m, vy, sh.damp_den);
-                fz = damped(fz, sh.eta, m, vz, sh.damp_den);
+        if (MIXED && npass > 0) {
+#pragma unroll
+            for (int r = 0; r < MAX_TILE_CHUNKS; ++r) {
+                const int i = (int)threadIdx.x + r * B;
+                if (i < nn) {
+                    fx[r] = lsum[3 * i];
+                    fy[r] = lsum[3 * i + 1];
+                    fz[r] = lsum[3 * i + 2];
+                }
             }
-            if (k.nd.clamp[n]) fx = fy = fz = 0.0;
+        }
+        // each node's update, chunk by chunk (the equilibrium maxima are
+        // per-chunk partials)
+#pragma unroll
+        for (int r = 0; r < MAX_TILE_CHUNKS; ++r) {
+            if (r * B >= nn) break;  // block-uniform
+            const int i = (int)threadIdx.x + r * B;
+            const int n = ch.x + i;
+            double fnorm = -1.0, vnorm = -1.0;
+            int fidx = INT_MAX, vidx = INT_MAX;
+            if (i < nn) {
+                double gx = fx[r], gy = fy[r], gz = fz[r];
+                const double* V = k.b.v[par];
+                double* Xo = k.b.x[par ^ 1];
+                double* Vo = k.b.v[par ^ 1];
+                double vx = V[n], vy = V[N + n], vz = V[2 * N + n];
+                const double m = k.nd.mass[n];
+                if (!sh.gd) {
+                    gx = damped(gx, sh.eta, m, vx, sh.damp_den);
+                    gy = damped(gy, sh.eta, m, vy, sh.damp_den);
+                    gz = damped(gz, sh.eta, m, vz, sh.damp_den);
+                }
+                if (k.nd.clamp[n]) gx = gy = gz = 0.0;
+                if (do_check) {
+                    fnorm = norm3(gx, gy, gz);
+                    vnorm = norm3(vx, vy, vz);
+                    fidx = vidx = n;
+                }
+                const double lim = sh.lim;
+                if (!(fabs(gx) < lim) || !(fabs(gy) < lim) || !(fabs(gz) < lim))
+                    atomicOr(&k.shells[ch.z].failed, 1);
+                double x = lpos[3 * i], y = lpos[3 * i + 1], z = lpos[3 * i + 2];
+                update(sh, m, gx, vx, x);
+                update(sh, m, gy, vy, y);
+                update(sh, m, gz, vz, z);
+                Xo[n] = x;
+                Xo[N + n] = y;
+                Xo[2 * N + n] = z;
+                Vo[n] = vx;
+                Vo[N + n] = vy;
+                Vo[2 * N + n] = vz;
+            }
             if (do_check) {
-                fnorm = norm3(fx, fy, fz);
-                vnorm = norm3(vx, vy, vz);
-                fidx = vidx = n;
-            }
-            const double lim = sh.lim;
-            if (!(fabs(fx) < lim) || !(fabs(fy) < lim) || !(fabs(fz) < lim)) atomicOr(&k.shells[ch.z].failed, 1);
-            const int i = threadIdx.x;
-            double x = lpos[3 * i], y = lpos[3 * i + 1], z = lpos[3 * i + 2];
-            update(sh, m, fx, vx, x);
-            update(sh, m, fy, vy, y);
-            update(sh, m, fz, vz, z);
-            Xo[n] = x;
-            Xo[N + n] = y;
-            Xo[2 * N + n] = z;
-            Vo[n] = vx;
-            Vo[N + n] = vy;
-            Vo[2 * N + n] = vz;
-        }
-        if (do_check) {
-            double v2[2] = {fnorm, vnorm};
-            int i2[2] = {fidx, vidx};
-            block_argmax<2>(v2, i2);
-            if (threadIdx.x == 0) {
-                k.b.part[2 * ci] = v2[0];
-                k.b.part[2 * ci + 1] = v2[1];
-                k.b.part_i[2 * ci] = i2[0];
-                k.b.part_i[2 * ci + 1] = i2[1];
+                double v2[2] = {fnorm, vnorm};
+                int i2[2] = {fidx, vidx};
+                block_argmax<2>(v2, i2);
+                if (threadIdx.x == 0) {
+                    const int cc = ch.w + r;
+                    k.b.part[2 * cc] = v2[0];
+                    k.b.part[2 * cc + 1] = v2[1];
+                    k.b.part_i[2 * cc] = i2[0];
+                    k.b.part_i[2 * cc + 1] = i2[1];
+                }
             }
         }
         __syncthreads();  // the next tile restages the LDS
@@ -316,14 +336,12 @@ void launch_tile_variant(const Launch& L, const TileKArgs& k, size_t lds, bool c
         hipLaunchKernelGGL((tile_step_kernel<MIXED, false, LCE, UNI>), dim3(L.grid), dim3(L.block), lds, L.stream, k);
 }
 
-void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, const Chunk* chunks,
-                      int nchunks, ShellDev* shells, bool mixed, bool lce, bool uni_data, const StepFlags& fl) {
+void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, const TileData& tl, ShellDev* shells,
+                      bool mixed, bool lce, bool uni_data, const StepFlags& fl) {
     TileKArgs k{};
     k.nd = nd;
     k.b = b;
     k.tl = tl;
-    k.chunks = chunks;
-    k.nchunks = nchunks;
     k.shells = shells;
     k.fl = fl;
     const size_t lds = (size_t)tile_lds_bytes(tl, mixed);

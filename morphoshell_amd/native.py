@@ -112,7 +112,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("tri_block", C.c_int32), ("node_block", C.c_int32), ("tri_grid_per_cu", C.c_int32),
         ("node_grid_per_cu", C.c_int32), ("graph_steps", C.c_int32), ("fused", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("tile_chunks", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -316,18 +316,20 @@ def test_fused_equilibrium_check_and_halt():
 
 # ------------------------------------------------- fused owner-computes step
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("node_block,grid", [(64, 0), (128, 2), (256, 0)])
-def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid):
+@pytest.mark.parametrize("node_block,grid,tile_chunks", [(64, 0, 1), (128, 2, 1), (256, 0, 1), (64, 0, 2),
+                                                         (256, 3, 2)])
+def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid, tile_chunks):
     """The fused tile launch sums the same contributions in the same
     patch-incidence order as the two-launch path: states agree bit for bit,
-    including the equilibrium maxima of checked steps."""
+    including the equilibrium maxima of checked steps (per-chunk partials,
+    also when a tile spans two chunks)."""
     x, v = deformed_state(sh, seed=40)
     with _ctx(sh, precision) as a, _ctx(sh, precision) as b:
         a.set_state(0, x, v)
         b.set_state(0, x, v)
         a.set_tuning(node_block=node_block, node_grid_per_cu=grid)
-        b.set_tuning(node_block=node_block, node_grid_per_cu=grid, fused=1)
-        assert b.get_tuning()["fused"] == 1
+        b.set_tuning(node_block=node_block, node_grid_per_cu=grid, fused=1, tile_chunks=tile_chunks)
+        assert b.get_tuning()["fused"] == 1 and b.get_tuning()["tile_chunks"] == tile_chunks
         ra = a.step(41, equil_every=10, force_tol=1e-30, speed_tol=1e-30)
         rb = b.step(41, equil_every=10, force_tol=1e-30, speed_tol=1e-30)
         xa, va = a.get_state(0)
@@ -341,11 +343,12 @@ def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid):
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
 
 
-def test_fused_tile_step_batch_matches_oracle():
+@pytest.mark.parametrize("tile_chunks", [1, 2])
+def test_fused_tile_step_batch_matches_oracle(tile_chunks):
     shells = [_shell(0.08, seed=1), _shell(0.05, seed=2, kappa=-1.0, clamp_every=7), _shell(0.11, seed=3, h=0.02)]
     states = [deformed_state(s, seed=50 + i) for i, s in enumerate(shells)]
     with _ctx(shells) as ctx, _ctx(shells) as ref:
-        ctx.set_tuning(node_block=128, fused=1)
+        ctx.set_tuning(node_block=128, fused=1, tile_chunks=tile_chunks)
         for i, st in enumerate(states):
             ctx.set_state(i, *st)
             ref.set_state(i, *st)
@@ -551,15 +554,15 @@ def test_annulus_matches_oracle_in_both_step_forms():
     assert np.abs(xg - xo).max() <= 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_many_tiny_shells_fused_equals_two_launch(precision):
+@pytest.mark.parametrize("precision,tile_chunks", [("fp64", 1), ("mixed", 1), ("fp64", 2), ("mixed", 2)])
+def test_many_tiny_shells_fused_equals_two_launch(precision, tile_chunks):
     """A ragged batch of 40 shells of 30-400 triangles (most smaller than one
     tile): the fused step equals the two-launch step bit for bit."""
     shells = [_shell(0.16 + 0.25 * (i % 5) / 4, seed=i, h=0.02 + 0.005 * (i % 3), kappa=0.3 * (i % 4) - 0.4)
               for i in range(40)]
     states = [deformed_state(sh, seed=100 + i) for i, sh in enumerate(shells)]
     with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
-        b.set_tuning(node_block=256, fused=1)
+        b.set_tuning(node_block=64, fused=1, tile_chunks=tile_chunks)
         for i, st in enumerate(states):
             a.set_state(i, *st)
             b.set_state(i, *st)
