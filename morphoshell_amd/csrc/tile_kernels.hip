@@ -80,14 +80,13 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
 // the FP64 kernel keeps them in registers, where its larger LDS tile would
 // otherwise cost a workgroup per CU), then contributions (3 CT per pass
 // entry).
-// Waves per SIMD asked of the compiler (their LDS tiles allow them): without
-// the LCE protocol 2 in FP64 and 4 in mixed mode; with it, 3 in mixed mode
-// and what fits in FP64.  Each triangle's record is loaded whole before its
-// first use; at 2 waves the FP64 kernel keeps every wave's record in flight
-// without spilling, and measured faster than at 3 waves (168 registers, where
-// the loads had to wait for registers).
+// Waves per SIMD asked of the compiler (their LDS tiles allow them): 2 in
+// FP64 and 4 in mixed mode (3 with the LCE protocol).  Each triangle's record
+// is loaded whole before its first use; at 2 waves the FP64 kernel keeps
+// every wave's record in flight without spilling, and measured faster than at
+// 3 waves (168 registers, where the loads had to wait for registers).
 template <bool MIXED, bool CHECK, bool LCE, bool UNI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 1) : (MIXED ? 4 : 2))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 2) : (MIXED ? 4 : 2))))
 void tile_step_kernel(TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
     extern __shared__ double smem[];
