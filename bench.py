@@ -336,6 +336,9 @@ def bench_batch(args) -> dict:
         t_relax = time.perf_counter() - t1
     done = sum(1 for r in reps if r["status"] == 2)
     failed = sum(1 for r in reps if r["failed"])
+    # where the bounded run leaves the shells: residual force of each one's last check
+    fr = np.array([r["max_force_ratio"] for r in reps if not r["failed"]], float)
+    at_tol = {f"{t:g}": int(np.sum(fr <= t)) for t in (1e-3, 1e-4, 1e-5, 1e-6)}
     kb = kernel_bytes(T, N, True, uni)
     return {
         "metric": "triangle force-evals/s (ragged batch)",
@@ -357,7 +360,9 @@ def bench_batch(args) -> dict:
         "tri_record_bytes": tri_bytes(True, uni),
         "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
                   "shells_failed": failed, "shells_per_s": done / t_relax if t_relax > 0 else 0.0,
-                  "tolerances": {"force": 1e-6, "speed": 1e-5}},
+                  "tolerances": {"force": 1e-6, "speed": 1e-5},
+                  "shells_with_last_force_ratio_at_most": at_tol,
+                  "last_force_ratio_median": float(np.median(fr)) if fr.size else None},
         "setup_s": t_setup,
     }
 
