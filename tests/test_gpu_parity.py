@@ -678,3 +678,22 @@ def test_per_triangle_thickness_modulus_and_bbar(precision):
         xo, vo = xs[0][0].copy(), xs[0][1].copy()
         assert orc.System.from_shell(varied).run_steps(xo, vo, 20) == -1
         assert np.abs(ends[1][0] - xo).max() <= 1e-11
+
+
+def test_fused_tiles_survive_a_precision_switch(sh):
+    """Tiles are built with the pass length of the context's precision at
+    the time (2B triangles in FP64, B in mixed mode); switching precision on
+    a live fused context keeps them, and the states equal those of a context
+    built in the new precision, bit for bit, both ways round."""
+    x, v = deformed_state(sh, seed=91)
+    for first, second in (("fp64", "mixed"), ("mixed", "fp64")):
+        with _ctx(sh, first) as a, _ctx(sh, second) as b:
+            a.set_tuning(node_block=64, fused=1)
+            b.set_tuning(node_block=64, fused=1)
+            a.set_precision(second)
+            for c in (a, b):
+                c.set_state(0, x, v)
+            ra = a.step(23, equil_every=5, force_tol=1e-30, speed_tol=1e-30)
+            rb = b.step(23, equil_every=5, force_tol=1e-30, speed_tol=1e-30)
+            assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0]), (first, second)
+            assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
