@@ -128,18 +128,8 @@ void tile_step_kernel(TileKArgs k) {
         for (int pass = 0; pass < npass; ++pass) {
             // 2. this pass's triangles into LDS
             const int ja = j0 + pass * split, jb = min(j1, ja + split);
-            for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
-                // opaque to loop strength reduction: the record loads keep one
-                // 32-bit lane offset against scalar stream bases instead of a
-                // 64-bit induction pointer per stream
-                unsigned t = (unsigned)j;
-                asm volatile("" : "+v"(t));
-                // six 16-bit ids per entry, read as three 32-bit words
-                const unsigned* lw = reinterpret_cast<const unsigned*>(k.tl.lpid);
-                const unsigned w0 = ld(lw, 3 * t), w1 = ld(lw, 3 * t + 1), w2 = ld(lw, 3 * t + 2);
-                const unsigned lp[6] = {w0 & 0xFFFFu, w0 >> 16, w1 & 0xFFFFu, w1 >> 16, w2 & 0xFFFFu, w2 >> 16};
-                const TilePos pos{lpos, lp};
-                // this wave's record block (its 64 entries start at a multiple of 64)
+            // the record block of entry j (a wave's 64 entries start at a multiple of 64)
+            auto view = [&](int j) {
                 const size_t blk = (size_t)uni(j >> 6) * (REC_SLOTS * 64);
                 const double* r64 = k.tl.rec64 + blk;
                 const float* r32 = k.tl.rec32 + blk;
@@ -157,12 +147,28 @@ void tile_step_kernel(TileKArgs k) {
                 rv.G32 = r32 + RS_G * 64;
                 rv.H32 = r32 + RS_H * 64;
                 rv.hmuA32 = r32 + RS_HMUA * 64;
-                const unsigned lane = (unsigned)j & 63u;
-                CT out[18];
-                if (MIXED)
-                    tri_forces_mixed<LCE, UNI, 64>(rv, tp, lane, pos, reinterpret_cast<float*>(out));
-                else
-                    tri_forces_fp64<LCE, UNI, true, 64>(rv, tp, lane, pos, reinterpret_cast<double*>(out));
+                return rv;
+            };
+            // entry j's tile-local patch ids (six 16-bit ids read as three 32-bit words)
+            auto patch = [&](int j, unsigned lp[6]) {
+                // opaque to loop strength reduction: the loads keep one 32-bit
+                // lane offset against scalar bases instead of a 64-bit induction
+                // pointer per stream
+                unsigned t = (unsigned)j;
+                asm volatile("" : "+v"(t));
+                const unsigned* lw = reinterpret_cast<const unsigned*>(k.tl.lpid);
+                const unsigned w0 = ld(lw, 3 * t), w1 = ld(lw, 3 * t + 1), w2 = ld(lw, 3 * t + 2);
+                lp[0] = w0 & 0xFFFFu;
+                lp[1] = w0 >> 16;
+                lp[2] = w1 & 0xFFFFu;
+                lp[3] = w1 >> 16;
+                lp[4] = w2 & 0xFFFFu;
+                lp[5] = w2 >> 16;
+            };
+            // entry j's contributions to the tile's nodes, at their pass-local LDS positions
+            auto store = [&](int j, const CT out[18]) {
+                unsigned t = (unsigned)j;
+                asm volatile("" : "+v"(t));
                 const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
                 const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
                 const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
@@ -174,6 +180,27 @@ void tile_step_kernel(TileKArgs k) {
                         lcon[3 * q + 1] = out[3 * s + 1];
                         lcon[3 * q + 2] = out[3 * s + 2];
                     }
+                }
+            };
+            if constexpr (MIXED) {
+                for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
+                    unsigned lp[6];
+                    patch(j, lp);
+                    const TilePos pos{lpos, lp};
+                    float out[18];
+                    tri_forces_mixed<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, pos, out);
+                    store(j, reinterpret_cast<const CT*>(out));
+                }
+            } else {
+                // one entry at a time: a second record in flight (entries j and
+                // j + B together) needs more than the 256 registers of 2 waves
+                // per SIMD and measured 25% slower
+                for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
+                    unsigned lp[6];
+                    patch(j, lp);
+                    double out[18];
+                    tri_forces_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, TilePos{lpos, lp}, out);
+                    store(j, reinterpret_cast<const CT*>(out));
                 }
             }
             __syncthreads();

@@ -233,28 +233,38 @@ struct TilePos {
     }
 };
 
-// FP64 force contributions of triangle t (out[3*slot + c]); pos fetches its
-// six patch nodes.  EARLY issues the whole record (patch coefficients, dial
-// data, h, mu, area) before any use, one memory round trip per triangle;
-// without it the loads sit at their uses (fewer registers live).  Same
-// arithmetic either way.
-template <bool LCE, bool UNI, bool EARLY, int CS, class Pos>
-__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
-                                                double out[18]) {
-#pragma clang fp contract(on)
-    const size_t T = CS ? (size_t)CS : (size_t)td.T;
+// The FP64 record of triangle t: patch coefficients, dial data, h, mu and
+// area.  Loaded whole before any use, so that a triangle's record costs one
+// memory round trip (uniform branches, e.g. dial from ansatz, would otherwise
+// split the loads into serial round trips), and so that a caller can have
+// the next triangle's record in flight while it evaluates this one.
+struct RecF64 {
     double gc[6], hc[18], h, mu, A;
     DialRec dr;
-    if constexpr (EARLY) {
+};
+
+template <bool LCE, bool UNI, int CS = 0>
+__device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams& sp, unsigned t, RecF64& r) {
+    const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) gc[k] = ld(td.G + k * T, t);
+    for (int k = 0; k < 6; ++k) r.gc[k] = ld(td.G + k * T, t);
 #pragma unroll
-        for (int k = 0; k < 18; ++k) hc[k] = ld(td.H + k * T, t);
-        h = UNI ? sp.h : ld(td.h, t);
-        mu = UNI ? sp.mu : ld(td.mu, t);
-        A = ld(td.area, t);
-        load_dial<LCE, UNI, CS>(td, sp, t, dr);
-    }
+    for (int k = 0; k < 18; ++k) r.hc[k] = ld(td.H + k * T, t);
+    r.h = UNI ? sp.h : ld(td.h, t);
+    r.mu = UNI ? sp.mu : ld(td.mu, t);
+    r.A = ld(td.area, t);
+    load_dial<LCE, UNI, CS>(td, sp, t, r.dr);
+}
+
+// FP64 force contributions (out[3*slot + c]) of the triangle whose record is
+// r; pos fetches its six patch nodes.
+template <bool LCE, class Pos>
+__device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriParams& sp, const Pos& pos,
+                                                    double out[18]) {
+#pragma clang fp contract(on)
+    const double* gc = r.gc;
+    const double* hc = r.hc;
+    const double h = r.h, mu = r.mu, A = r.A;
     const double nu = sp.nu;
     double Px[6], Py[6], Pz[6];
 #pragma unroll
@@ -262,8 +272,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
     double Rx[3] = {0.0, 0.0, 0.0}, Ry[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-        const double g0 = EARLY ? gc[v] : ld(td.G + v * T, t);
-        const double g1 = EARLY ? gc[3 + v] : ld(td.G + (3 + v) * T, t);
+        const double g0 = gc[v], g1 = gc[3 + v];
         Rx[0] += g0 * Px[v];
         Rx[1] += g0 * Py[v];
         Rx[2] += g0 * Pz[v];
@@ -277,7 +286,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
         double sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const double hv = EARLY ? hc[s * 3 + kk] : ld(td.H + (s * 3 + kk) * T, t);
+            const double hv = hc[s * 3 + kk];
             sx += hv * Px[s];
             sy += hv * Py[s];
             sz += hv * Pz[s];
@@ -299,14 +308,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
     double ab[3], bb[3];
-    if constexpr (EARLY) {
-        dial_targets<LCE>(dr, sp, ab, bb);
-    } else {
-        targets<LCE, UNI, CS>(td, sp, t, ab, bb);
-        h = UNI ? sp.h : ld(td.h, t);
-        mu = UNI ? sp.mu : ld(td.mu, t);
-        A = ld(td.area, t);
-    }
+    dial_targets<LCE>(r.dr, sp, ab, bb);
     const double P[3] = {ab[2], -ab[1], ab[0]};
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
     const double idet32 = 1.0 / (det * sqrt(det));
@@ -340,20 +342,23 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const double ws = EARLY ? hc[s * 3] * Wb[0] + hc[s * 3 + 1] * Wb[1] + hc[s * 3 + 2] * Wb[2]
-                                : ld(td.H + (s * 3) * T, t) * Wb[0] + ld(td.H + (s * 3 + 1) * T, t) * Wb[1] +
-                                      ld(td.H + (s * 3 + 2) * T, t) * Wb[2];
+        const double ws = hc[s * 3] * Wb[0] + hc[s * 3 + 1] * Wb[1] + hc[s * 3 + 2] * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double f = ws * An[c];
-            if (s < 3) {
-                const double gx = EARLY ? gc[s] : ld(td.G + s * T, t);
-                const double gy = EARLY ? gc[3 + s] : ld(td.G + (3 + s) * T, t);
-                f -= gx * Agx[c] + gy * Agy[c];
-            }
+            if (s < 3) f -= gc[s] * Agx[c] + gc[3 + s] * Agy[c];
             out[3 * s + c] = f;
         }
     }
+}
+
+// FP64 force contributions of triangle t: its record, then the mechanics.
+template <bool LCE, bool UNI, int CS, class Pos>
+__device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
+                                                double out[18]) {
+    RecF64 r;
+    load_rec_fp64<LCE, UNI, CS>(td, sp, t, r);
+    tri_forces_fp64_rec<LCE>(r, sp, pos, out);
 }
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
