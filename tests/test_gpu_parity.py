@@ -479,6 +479,9 @@ def test_create_and_call_argument_errors(sh):
             ctx.set_tuning(fused=3)
         with pytest.raises(NativeError, match="node_block"):
             ctx.set_tuning(node_block=96)
+        with pytest.raises(NativeError, match="tile_chunks"):
+            ctx.set_tuning(tile_chunks=3)
+        assert ctx.get_tuning()["tile_chunks"] == 1
         with pytest.raises(NativeError, match="nsteps"):
             ctx.step(-1)
         # the context still works after refused calls
