@@ -271,14 +271,30 @@ def training_family(num: int, seed: int, tmin: int = 1000, tmax: int = 6000, fam
 def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int = 2000, precision: str = "fp64",
                 poll: int = 5000, equil_interval: int = 100):
     """Relax a batch with the device-run protocol.  plans[i] = (start dofs,
-    dial_start, steps per dial phase) or None for the flat start with noise
-    dialled from 0 over dial_steps.  Returns (reports, end dofs, seconds)."""
+    dial_start, steps per dial phase[, from_ansatz]) or None for the flat
+    start with noise dialled from 0 over dial_steps.  With from_ansatz the
+    shell is dialled from the start shape's own geometry (the reference's
+    dialling_from_ansatz_rather_than_ref_state, do_dialling.cpp:194-228): its
+    a and b are the targets at dial factor 0, so the start carries no elastic
+    force, and the programmed ones at 1.  Returns (reports, end dofs, seconds)."""
+    import copy
+    import dataclasses
+
     from .shell import Context
     from .workloads import flat_noisy_state
 
+    def ansatz(i):
+        return plans is not None and plans[i] is not None and len(plans[i]) > 3 and plans[i][3]
+
+    run = []
+    for i, sh in enumerate(shells):
+        if ansatz(i):
+            sh = copy.copy(sh)
+            sh.settings = dataclasses.replace(sh.settings, dialling_from_ansatz_rather_than_ref_state=True)
+        run.append(sh)
     t0 = time.perf_counter()
-    with Context(shells, precision=precision) as ctx:
-        ctx.set_tuning(node_block=128, fused=1)  # small shells: more, smaller tiles
+    with Context(run, precision=precision) as ctx:
+        starts = []
         for i, sh in enumerate(shells):
             plan = None if plans is None else plans[i]
             if plan is None:
@@ -288,6 +304,12 @@ def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int 
                 x, v = plan[0], None
                 f0, K = plan[1], plan[2]
             ctx.set_state(i, x, v)
+            if ansatz(i):
+                a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(i)
+                ctx.set_zero_dial_geometry(i, a.reshape(-1), b.reshape(-1))
+            starts.append((f0, K))
+        ctx.set_tuning(node_block=128, fused=1)  # small shells: more, smaller tiles
+        for i, (f0, K) in enumerate(starts):
             ctx.relax_setup(i, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=f0,
                             equil_interval=equil_interval, tol_f=tol, tol_v=tol)
         reps = ctx.relax_run(max_steps, poll_every=poll)
@@ -388,12 +410,14 @@ ALPHAS = (1.0, 0.75, 0.5, 0.35, 0.25, 0.15)
 SIGNS = (1.0, -1.0)
 
 
-def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000) -> tuple[list, list]:
+def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000, start: str = "ansatz") -> tuple[list, list]:
     """Start plans with the energy fallback.  For each shell the predicted
     displacement, in either z orientation, is scaled by the alpha of least
-    elastic energy (at dial factor 1); the scaled prediction starts at dial
-    factor 1 when its energy is below that of the flat start with noise,
-    otherwise the flat start is dialled as usual.  Returns (plans, the chosen
+    elastic energy (at dial factor 1); the scaled prediction is used when its
+    energy is below that of the flat start with noise, otherwise the flat
+    start is dialled as usual.  start = "direct": the prediction starts at
+    dial factor 1; "ansatz": it is dialled from its own geometry over
+    dial_steps (no elastic force at the start).  Returns (plans, the chosen
     signed alpha or None)."""
     from .workloads import flat_noisy_state
 
@@ -415,19 +439,25 @@ def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000) -> tuple
         e = np.asarray(e_pred[i])
         k = int(np.nanargmin(e)) if np.isfinite(e).any() else -1
         ok = k >= 0 and e[k] < e_flat[i]
-        plans.append((cands[i][k], 1.0, 1) if ok else None)
+        if not ok:
+            plans.append(None)
+        elif start == "ansatz":
+            plans.append((cands[i][k], 0.0, dial_steps, True))
+        else:
+            plans.append((cands[i][k], 1.0, 1))
         used.append(labels[k][0] * labels[k][1] if ok else None)
     return plans, used
 
 
 def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: int = 2000, tol: float = 1e-5,
-             max_steps: int = 600000, family: str = "defect") -> dict:
-    """Steps to equilibrium from the learned warm start (at dial factor 1,
+             max_steps: int = 600000, family: str = "defect", start: str = "ansatz") -> dict:
+    """Steps to equilibrium from the learned warm start (started at dial
+    factor 1, or dialled from its own geometry: start, see warm_start_plans;
     with the fallback) against the flat start with noise dialled as usual, on
     unseen seeds; final energies of both at dial factor 1."""
     shells = training_family(num, seed, family=family)
     t0 = time.perf_counter()
-    plans, used = warm_start_plans(net, shells, dial_steps)
+    plans, used = warm_start_plans(net, shells, dial_steps, start=start)
     t_pred = time.perf_counter() - t0
     base, base_end, t_base = relax_batch(shells, tol, max_steps, dial_steps=dial_steps)
     warm, warm_end, t_warm = relax_batch(shells, tol, max_steps, plans=plans, dial_steps=dial_steps)
@@ -454,6 +484,7 @@ def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: i
         "final_energy_rel_diff_median": f(np.median(de)) if len(de) else None,
         "final_energy_lower_or_equal_fraction": f(np.mean(de <= 1e-6)) if len(de) else None,
         "tolerance": tol,
+        "start": start,
     }
 
 
@@ -472,10 +503,12 @@ def main(argv=None) -> int:
     ap.add_argument("--data", default="", help="npz of generated equilibria: loaded if present, else written")
     ap.add_argument("--family", default="defect", choices=FAMILIES)
     ap.add_argument("--params", default="", help="trained parameters to evaluate (skips generation and training)")
+    ap.add_argument("--start", default="ansatz", choices=("direct", "ansatz"),
+                    help="warm start at dial factor 1, or dialled from the prediction's own geometry")
     args = ap.parse_args(argv)
     if args.params:
         net = WarmStartNet.load(args.params)
-        ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family)
+        ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start)
         print(json.dumps({"params": args.params, "eval": ev}), flush=True)
         return 0
     t0 = time.time()
@@ -493,7 +526,7 @@ def main(argv=None) -> int:
     hist = train(net, data, epochs=epochs, lr=args.lr)
     t_train = time.time() - t0
     net.save(args.out)
-    ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family)
+    ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start)
     print(json.dumps({"equilibria": len(data), "generate_s": t_gen, "train_s": t_train,
                       "loss_first": hist[0], "loss_last": hist[-1], "eval": ev}), flush=True)
     return 0

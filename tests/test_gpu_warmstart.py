@@ -124,3 +124,36 @@ def test_trained_warm_start_lowers_the_start_energy():
         assert (ew < ef) if p else (ew == ef)
     reps, _, _ = ws.relax_batch(shells, 1e-4, 400_000, plans=plans)
     assert sum(r["status"] == 2 for r in reps) >= 6
+
+
+def test_warm_start_dialled_from_its_own_geometry_starts_force_free():
+    """The warm start's ansatz mode (dialling_from_ansatz_rather_than_ref_state,
+    do_dialling.cpp:194-228): the start shape's own a and b are the targets at
+    dial factor 0, so the elastic forces there vanish to rounding, and at
+    factor 1 they are the programmed metric's forces."""
+    import dataclasses
+
+    from conftest import deformed_state, grf_disc
+    from morphoshell_amd.shell import Context, Settings, prepare_shell
+
+    m = grf_disc(0.07, seed=5, h=0.02, kappa=0.4)
+    plain = prepare_shell(m, Settings())
+    sh = prepare_shell(m, dataclasses.replace(Settings(), dialling_from_ansatz_rather_than_ref_state=True))
+    x, v = deformed_state(sh, seed=6)
+    with Context(sh) as ctx, Context(plain) as ref:
+        ctx.set_state(0, x, v)
+        a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
+        ctx.set_zero_dial_geometry(0, a.reshape(-1), b.reshape(-1))
+        ctx.set_dial_clock(0, 0.0, 1.0, 1000, 0, True)
+        ctx.calc_deformation_forces()
+        F0 = ctx.get_forces(0)
+        ctx.set_dial_clock(0, 0.0, 1.0, 1000, 1000, True)
+        ctx.calc_deformation_forces()
+        F1 = ctx.get_forces(0)
+        ref.set_state(0, x, v)
+        ref.calc_deformation_forces()
+        Fr = ref.get_forces(0)
+    scale = np.abs(Fr).max()
+    print(f"force at dial factor 0: {np.abs(F0).max() / scale:.2e} of the programmed forces")
+    assert np.abs(F0).max() <= 1e-9 * scale
+    assert np.abs(F1 - Fr).max() <= 1e-12 * scale
