@@ -153,6 +153,38 @@ def cpu_baseline(sh, x, v, budget_s: float) -> dict:
     }
 
 
+def baseline_parity(sh, x, v, n: int = 20) -> dict:
+    """Part of the CPU-baseline leg: the device's forces (both modes), FP64
+    energies and an n-step fused FP64 trajectory at the benchmark state,
+    against the CPU baseline's own results for the same state."""
+    from morphoshell_amd.shell import Context
+    from oracle import oracle as orc
+
+    parity = {}
+    sysm = orc.System.from_shell(sh)
+    Fo = sysm.deformation_forces(x)
+    scale = np.abs(Fo).max()
+    E = None
+    for prec in ("fp64", "mixed"):
+        with Context(sh, precision=prec) as ctx:
+            ctx.set_state(0, x, v)
+            ctx.calc_deformation_forces()
+            parity[f"{prec}_force_rel_err"] = float(np.abs(ctx.get_forces(0) - Fo).max() / scale)
+            if prec == "fp64":
+                E = ctx.calc_energies(0)
+    Eo = sysm.energies(x, v)
+    parity["fp64_energy_rel_err"] = float(np.abs(E[:2] - Eo[:2]).max() / max(abs(Eo[:2]).max(), 1e-300))
+    with Context(sh, precision="fp64") as ctx:
+        ctx.set_tuning(node_block=256, fused=1)
+        ctx.set_state(0, x, v)
+        ctx.step(n)
+        xg = ctx.get_state(0)[0]
+    xo, vo = x.copy(), v.copy()
+    sysm.run_steps(xo, vo, n)
+    parity[f"fp64_fused_{n}_step_position_max_abs_err"] = float(np.abs(xg - xo).max())
+    return parity
+
+
 # ------------------------------------------------------------------ single
 
 def bench_single(args) -> dict:
@@ -189,38 +221,14 @@ def bench_single(args) -> dict:
         read_gbs = (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
         log(f"copy kernel 1 GiB: {copy_gbs:.0f} GB/s; read-only stream 1 GiB: {read_gbs:.0f} GB/s")
 
-    parity = {}
-    if not args.no_parity:
-        from oracle import oracle as orc
-
-        sysm = orc.System.from_shell(sh)
-        Fo = sysm.deformation_forces(x)
-        scale = np.abs(Fo).max()
-        for prec in ("fp64", "mixed"):
-            with Context(sh, precision=prec) as ctx:
-                ctx.set_state(0, x, v)
-                ctx.calc_deformation_forces()
-                parity[f"{prec}_force_rel_err"] = float(np.abs(ctx.get_forces(0) - Fo).max() / scale)
-                E = ctx.calc_energies(0)
-        Eo = sysm.energies(x, v)
-        parity["fp64_energy_rel_err"] = float(np.abs(E[:2] - Eo[:2]).max() / max(abs(Eo[:2]).max(), 1e-300))
-        # a fused-form trajectory against the oracle's
-        n = 20
-        with Context(sh, precision="fp64") as ctx:
-            ctx.set_tuning(node_block=256, fused=1)
-            ctx.set_state(0, x, v)
-            ctx.step(n)
-            xg = ctx.get_state(0)[0]
-        xo, vo = x.copy(), v.copy()
-        sysm.run_steps(xo, vo, n)
-        parity[f"fp64_fused_{n}_step_position_max_abs_err"] = float(np.abs(xg - xo).max())
-        log(f"parity {parity}")
-
-    cpu = None
+    cpu, parity = None, {}
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(sh, x, v, args.cpu_budget)
         log(f"cpu {cpu['value'] / 1e6:.2f} M tri-evals/s on {cpu['cores']} threads")
+        if not args.no_parity:
+            parity = baseline_parity(sh, x, v)
+            log(f"parity {parity}")
 
     def summary(prec, form):
         r = res.get((prec, form))
@@ -400,7 +408,7 @@ def bench_relax_large(args) -> dict:
             while True:
                 r = ctx.relax_run(chunk, poll_every=chunk)[0]
                 # the residual of the last equilibrium check (none while dialling)
-                f = float(r["max_force_ratio"]) if r["status"] >= 1 else float("nan")
+                f = float(r["max_force_ratio"]) if r["status"] >= 1 and r["max_force_ratio"] > 0 else float("nan")
                 pts.append((time.perf_counter() - t0, int(r["steps"]), f))
                 if r["halted"] or r["failed"] or r["steps"] >= steps:
                     break
@@ -434,6 +442,133 @@ def bench_relax_large(args) -> dict:
                       "energy": energy[p]} for p in series},
         "series": {p: [[round(t, 4), n, None if np.isnan(f) else f] for t, n, f in series[p]] for p in series},
     }
+
+
+def _relax_series(sh, precision, x, v, K, steps, chunk, from_ansatz=False):
+    """Relax one shell under the device-run protocol (dialled over K steps,
+    from its start geometry when from_ansatz) for a bounded number of steps;
+    (wall seconds, steps, residual of the last check or nan) every chunk
+    steps, the end state and its FP64 energies."""
+    import copy
+    import dataclasses
+
+    from morphoshell_amd.shell import Context
+
+    if from_ansatz:
+        sh = copy.copy(sh)
+        sh.settings = dataclasses.replace(sh.settings, dialling_from_ansatz_rather_than_ref_state=True)
+    with Context(sh, precision=precision) as ctx:
+        ctx.set_state(0, x, v)
+        if from_ansatz:
+            a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
+            ctx.set_zero_dial_geometry(0, a.reshape(-1), b.reshape(-1))
+        ctx.set_tuning(node_block=256, fused=1)
+        ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=1000,
+                        tol_f=1e-6, tol_v=1e-5)
+        ctx.synchronize()
+        pts, t0 = [], time.perf_counter()
+        while True:
+            r = ctx.relax_run(chunk, poll_every=chunk)[0]
+            # the residual of the last equilibrium check (none while dialling,
+            # nor before the first check after it)
+            f = float(r["max_force_ratio"]) if r["status"] >= 1 and r["max_force_ratio"] > 0 else float("nan")
+            pts.append((time.perf_counter() - t0, int(r["steps"]), f))
+            if r["halted"] or r["failed"] or r["steps"] >= steps:
+                break
+        xe, ve = ctx.get_state(0)
+        E = ctx.calc_energies(0)
+    return pts, xe, ve, E, bool(r["failed"])
+
+
+def bench_c5(args) -> dict:
+    """The thin-sheet stress test with the chosen parts together, bounded:
+    a disc at spacing 0.001 (7.25M triangles), h = 0.002, a +1 defect metric
+    (final lambda 0.9) modulated by a random field (amplitude 0.05,
+    correlation length 0.1) and bbar = 1.5 I, dialled over 50 000 steps.
+    Plain FP64 from the flat start with noise against the learned warm start
+    (dialled from its own geometry) with mixed-precision dynamics, each run
+    for the same number of steps; a relaxation to 1e-6 F_char is far beyond
+    the bound.  The CPU-baseline leg checks the mixed end state against the
+    FP64 oracle (forces, energies)."""
+    from morphoshell_amd import warmstart as ws
+    from morphoshell_amd import workloads
+    from morphoshell_amd.meshing import defect_metric_abar_info, disc_mesh, grf, uniform_bbar_info
+    from morphoshell_amd.shell import Context, Settings, prepare_shell
+
+    t0 = time.time()
+    m = disc_mesh(args.c5_spacing, seed=0)
+    c = m.centroids()
+    g = np.clip(grf(c[:, :2], 0.1, 64, seed=5), -3, 3)
+    m.abar_info = defect_metric_abar_info(c, 0.9 * (1.0 + 0.05 * g), 0.5)
+    m.bbar_info = uniform_bbar_info(m.num_tris, 1.5)
+    m.fill_defaults(h=0.002, mu=1.0)
+    sh = prepare_shell(m, Settings(def_poisson_ratio=0.5))
+    t_setup = time.time() - t0
+    K, steps, chunk = 50_000, args.c5_steps, 5000
+    log(f"C5: T={sh.T} N={sh.N}, set-up {t_setup:.1f}s, dial over {K} steps, {steps} steps per run")
+    x0, v0 = workloads.flat_noisy_state(sh)
+    plain, x64, _, E64, fail64 = _relax_series(sh, "fp64", x0, v0, K, steps, chunk)
+    log(f"fp64 plain: {plain[-1][1]} steps in {plain[-1][0]:.1f}s, residual {plain[-1][2]:.3e}")
+    t0 = time.perf_counter()
+    pred_error = None
+    try:
+        net = ws.WarmStartNet.load(ws.PRETRAINED_DEFECT)
+        plans, used = ws.warm_start_plans(net, [sh], dial_steps=K, start="ansatz")
+        net.close()  # its activations (~36 edge-sized buffers) before the dynamics
+    except Exception as e:  # a failed prediction leaves the flat start
+        pred_error = f"{type(e).__name__}: {e}"
+        plans, used = [None], [None]
+        log(f"warm start prediction failed: {pred_error}")
+    t_pred = time.perf_counter() - t0
+    if plans[0] is not None:
+        warm, xm, vm, Em, failm = _relax_series(sh, "mixed", plans[0][0], None, K, steps, chunk, from_ansatz=True)
+    else:
+        warm, xm, vm, Em, failm = _relax_series(sh, "mixed", x0, v0, K, steps, chunk)
+    log(f"warm start ({'used, alpha %s' % used[0] if plans[0] is not None else 'fallback to flat'}, "
+        f"{t_pred:.1f}s) + mixed: {warm[-1][1]} steps in {warm[-1][0]:.1f}s, residual {warm[-1][2]:.3e}")
+    uni = uniform_data([sh])
+    ms64 = 1e3 * plain[-1][0] / max(1, plain[-1][1])
+    msm = 1e3 * warm[-1][0] / max(1, warm[-1][1])
+    out = {
+        "metric": "thin-sheet stress test, bounded",
+        "config": {"workload": f"disc, spacing {args.c5_spacing}, {sh.T} triangles, h=0.002, +1 defect lambda 0.9 "
+                               f"x (1 + 0.05 GRF(corr 0.1)), bbar = 1.5 I, dialled over {K} steps, {steps} steps "
+                               "per run"},
+        "setup_s": t_setup,
+        "fp64_plain": {"steps": plain[-1][1], "seconds": plain[-1][0], "ms_per_step": ms64,
+                       "last_force_ratio": plain[-1][2], "failed": fail64, "energy": float(E64[0] + E64[1]),
+                       "alg_GBs": step_bytes(sh.T, sh.N, False, uni) / (ms64 * 1e-3) / 1e9},
+        "warm_start_mixed": {"prediction_used": plans[0] is not None, "alpha": used[0], "predict_s": t_pred,
+                             "prediction_error": pred_error,
+                             "steps": warm[-1][1], "seconds": warm[-1][0], "ms_per_step": msm,
+                             "last_force_ratio": warm[-1][2], "failed": failm, "energy": float(Em[0] + Em[1]),
+                             "alg_GBs": step_bytes(sh.T, sh.N, True, uni) / (msm * 1e-3) / 1e9},
+        "energy_rel_diff_warm_mixed_vs_fp64": float((Em[0] + Em[1]) / (E64[0] + E64[1]) - 1.0),
+        "series": {"fp64_plain": plain, "warm_start_mixed": warm},
+    }
+    out["series"] = {k: [[round(t, 3), n, None if np.isnan(f) else f] for t, n, f in v]
+                     for k, v in out["series"].items()}
+    if not args.no_cpu:
+        # CPU-baseline leg: the FP64 oracle at the mixed run's end state
+        from oracle import oracle as orc
+
+        sysm = orc.System.from_shell(sh)
+        t0 = time.perf_counter()
+        Fo = sysm.deformation_forces(xm)
+        t_cpu = time.perf_counter() - t0
+        Eo = sysm.energies(xm, vm)
+        with Context(sh, precision="mixed") as ctx:
+            ctx.set_state(0, xm, vm)
+            ctx.calc_deformation_forces()
+            Fm = ctx.get_forces(0)
+            E = ctx.calc_energies(0)
+        scale = np.abs(Fo).max()
+        out["oracle_check_of_mixed_end_state"] = {
+            "mixed_force_rel_err": float(np.abs(Fm - Fo).max() / scale),
+            "energy_rel_err": float(abs((E[0] + E[1]) - (Eo[0] + Eo[1])) / abs(Eo[0] + Eo[1])),
+            "oracle_force_eval_s": t_cpu, "cores": orc.lib().orc_num_threads()}
+        log(f"oracle check {out['oracle_check_of_mixed_end_state']}")
+    return out
 
 
 def bench_relax(args) -> dict:
@@ -537,11 +672,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large"])
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large", "c5"])
     ap.add_argument("--spacing", type=float, default=0.002)
     ap.add_argument("--shells", type=int, default=512)
     ap.add_argument("--relax-steps", type=int, default=20000)
     ap.add_argument("--relax-large-steps", type=int, default=60000)
+    ap.add_argument("--c5-spacing", type=float, default=0.001)
+    ap.add_argument("--c5-steps", type=int, default=60000)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
@@ -555,7 +692,7 @@ def main() -> int:
               "runs on several GPUs are not part of it.", file=sys.stderr)
         return 2
     fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax,
-          "relax_large": bench_relax_large}[args.workload]
+          "relax_large": bench_relax_large, "c5": bench_c5}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

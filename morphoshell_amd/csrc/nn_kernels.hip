@@ -7,6 +7,8 @@
 // runs in a fixed order, so training is bitwise repeatable.
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 
@@ -38,7 +40,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __shared__ short As[BM][BK + 4];
     __shared__ short Bs[BK][BN + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    // row tiles on x (up to 2^31 - 1 blocks: edge GEMMs of large meshes have
+    // millions of rows), column tiles on y
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int z = blockIdx.z;
     const int kbeg = z * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
@@ -101,18 +105,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 
 // out[i] = sum_z part[z][i] (+ out[i] if accumulate), fixed order.
 __global__ void sum_partials_kernel(const float* part, int nz, long long n, float* out, int accumulate) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.f;
-    for (int z = 0; z < nz; ++z) s += part[(size_t)z * n + i];
-    out[i] = accumulate ? out[i] + s : s;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float s = 0.f;
+        for (int z = 0; z < nz; ++z) s += part[(size_t)z * n + i];
+        out[i] = accumulate ? out[i] + s : s;
+    }
 }
 
 __device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
 
 __global__ void silu_kernel(const float* x, float* y, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float v = x[i];
         y[i] = v * sigm(v);
     }
@@ -120,72 +125,77 @@ __global__ void silu_kernel(const float* x, float* y, long long n) {
 
 // dx = dy * silu'(x)
 __global__ void silu_bwd_kernel(const float* x, const float* dy, float* dx, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float v = x[i], s = sigm(v);
         dx[i] = dy[i] * (s + v * s * (1.f - s));
     }
 }
 
 __global__ void add_kernel(float* y, const float* x, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] += x[i];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] += x[i];
 }
 
 // out[e] = [h[dst[e]], h[src[e]], ef[e]]  (E x 3W)
 __global__ void edge_concat_kernel(const float* h, const float* ef, const int* src, const int* dst, int E, int W,
                                    float* out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)E * 3 * W) return;
-    const int e = (int)(i / (3 * W)), c = (int)(i % (3 * W));
-    float v;
-    if (c < W)
-        v = h[(size_t)dst[e] * W + c];
-    else if (c < 2 * W)
-        v = h[(size_t)src[e] * W + (c - W)];
-    else
-        v = ef[(size_t)e * W + (c - 2 * W)];
-    out[i] = v;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)E * 3 * W; i += stride) {
+        const int e = (int)(i / (3 * W)), c = (int)(i % (3 * W));
+        float v;
+        if (c < W)
+            v = h[(size_t)dst[e] * W + c];
+        else if (c < 2 * W)
+            v = h[(size_t)src[e] * W + (c - W)];
+        else
+            v = ef[(size_t)e * W + (c - 2 * W)];
+        out[i] = v;
+    }
 }
 
 // out[n] = [h[n], a[n]]  (N x 2W)
 __global__ void node_concat_kernel(const float* h, const float* a, int N, int W, float* out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)N * 2 * W) return;
-    const int n = (int)(i / (2 * W)), c = (int)(i % (2 * W));
-    out[i] = c < W ? h[(size_t)n * W + c] : a[(size_t)n * W + (c - W)];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)N * 2 * W; i += stride) {
+        const int n = (int)(i / (2 * W)), c = (int)(i % (2 * W));
+        out[i] = c < W ? h[(size_t)n * W + c] : a[(size_t)n * W + (c - W)];
+    }
 }
 
 // agg[n][c] = sum over edges e in [off[n], off[n+1]) of m[perm ? perm[e] : e][c]
 __global__ void segment_sum_kernel(const float* m, const int* off, const int* perm, int N, int W, int ld_in,
                                    int col0, float* out, int ld_out, int accumulate) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)N * W) return;
-    const int n = (int)(i / W), c = (int)(i % W);
-    float s = 0.f;
-    for (int e = off[n]; e < off[n + 1]; ++e) {
-        const int ee = perm ? perm[e] : e;
-        s += m[(size_t)ee * ld_in + col0 + c];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)N * W; i += stride) {
+        const int n = (int)(i / W), c = (int)(i % W);
+        float s = 0.f;
+        for (int e = off[n]; e < off[n + 1]; ++e) {
+            const int ee = perm ? perm[e] : e;
+            s += m[(size_t)ee * ld_in + col0 + c];
+        }
+        float* d = out + (size_t)n * ld_out + c;
+        *d = accumulate ? *d + s : s;
     }
-    float* d = out + (size_t)n * ld_out + c;
-    *d = accumulate ? *d + s : s;
 }
 
 // dm[e] = da[dst[e]]
 __global__ void gather_rows_kernel(const float* a, const int* idx, int E, int W, float* out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)E * W) return;
-    const int e = (int)(i / W), c = (int)(i % W);
-    out[i] = a[(size_t)idx[e] * W + c];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)E * W; i += stride) {
+        const int e = (int)(i / W), c = (int)(i % W);
+        out[i] = a[(size_t)idx[e] * W + c];
+    }
 }
 
 // copy columns [col0, col0 + W) of an (R x ld) array into (R x W), optionally accumulating
 __global__ void take_cols_kernel(const float* in, int R, int ld, int col0, int W, float* out, int accumulate) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)R * W) return;
-    const int r = (int)(i / W), c = (int)(i % W);
-    const float v = in[(size_t)r * ld + col0 + c];
-    out[i] = accumulate ? out[i] + v : v;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)R * W; i += stride) {
+        const int r = (int)(i / W), c = (int)(i % W);
+        const float v = in[(size_t)r * ld + col0 + c];
+        out[i] = accumulate ? out[i] + v : v;
+    }
 }
 
 // db[j] = sum_i dy[i][j]: block-per-column strided partial sums then a fixed
@@ -207,14 +217,15 @@ __global__ __launch_bounds__(256) void colsum_kernel(const float* dy, int R, int
 // Adam with bias correction (Kingma & Ba 2015).
 __global__ void adam_kernel(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
                             float eps, float c1, float c2, float wd) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i] + wd * p[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float gi = g[i] + wd * p[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
 }
 
 // Per-graph squared error with the z-reflection ambiguity: for graph g the
@@ -250,16 +261,18 @@ __global__ __launch_bounds__(256) void loss_terms_kernel(const float* pred, cons
 // d loss / d pred for loss = sum_g min(err+, err-) / (3 N_total)
 __global__ void loss_grad_kernel(const float* pred, const float* tgt, const int* gid, const float* terms, int N,
                                  float scale, float* dpred) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const int g = gid[n];
-    const float sgn = terms[2 * g + 1] < terms[2 * g] ? -1.f : 1.f;
-    dpred[3 * n] = 2.f * scale * (pred[3 * n] - tgt[3 * n]);
-    dpred[3 * n + 1] = 2.f * scale * (pred[3 * n + 1] - tgt[3 * n + 1]);
-    dpred[3 * n + 2] = 2.f * scale * (pred[3 * n + 2] - sgn * tgt[3 * n + 2]);
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        const int g = gid[n];
+        const float sgn = terms[2 * g + 1] < terms[2 * g] ? -1.f : 1.f;
+        dpred[3 * n] = 2.f * scale * (pred[3 * n] - tgt[3 * n]);
+        dpred[3 * n + 1] = 2.f * scale * (pred[3 * n + 1] - tgt[3 * n + 1]);
+        dpred[3 * n + 2] = 2.f * scale * (pred[3 * n + 2] - sgn * tgt[3 * n + 2]);
+    }
 }
 
-inline unsigned blocks(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+// Element-wise launches are grid-stride loops: at most 2^20 blocks, so a
+// launch never nears the 2^32 work-items a dispatch can hold.
+inline unsigned blocks(long long n, int b = 256) { return (unsigned)std::min<long long>((n + b - 1) / b, 1LL << 20); }
 
 }  // namespace
 
@@ -270,7 +283,7 @@ void gemm(hipStream_t s, const GemmArgs& a0, float* partials) {
     if (partials && a.ksplit > 1) zs = std::min(a.ksplit, kt);
     a.kchunk = ((kt + zs - 1) / zs) * BK;
     zs = (a.K + a.kchunk - 1) / a.kchunk;
-    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, zs);
+    dim3 grid((a.M + BM - 1) / BM, (a.N + BN - 1) / BN, zs);
     if (zs == 1) {
         hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, a);
         return;
