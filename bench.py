@@ -688,7 +688,7 @@ def main() -> int:
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (a shell, or a batch of shells, lives on one device); "
+        print("bench.py: runs on one GPU only (a shell, or a batch of shells, lives on one device); "
               "runs on several GPUs are not part of it.", file=sys.stderr)
         return 2
     fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax,

@@ -120,7 +120,7 @@ RunSettings read_run_settings(const std::string& path, bool ansatz_given) {
             "Error: The largest permitted value of dial_factors_to_hold_at_spacing is 1.0, because dial_factor never "
             "goes above 1.0, so anything larger would be meaningless.");
     if (st.slide_stiffness_prefactor > 0.0)
-        throw std::runtime_error("Error: glass slides (slide_stiffness_prefactor > 0) are not supported by this build.");
+        throw std::runtime_error("Error: glass slides (slide_stiffness_prefactor > 0) are not supported by ms_run.");
     return st;
 }
 
