@@ -700,3 +700,24 @@ def test_fused_tiles_survive_a_precision_switch(sh):
             rb = b.step(23, equil_every=5, force_tol=1e-30, speed_tol=1e-30)
             assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0]), (first, second)
             assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lce_shell_fused_equals_two_launch_while_dialling(precision):
+    """An LCE-protocol shell (protocol 3 targets, the kernel variant with the
+    pow() of the opto-thermal law) stepped while its dial factor advances:
+    the fused step equals the two-launch step bit for bit in both
+    precisions."""
+    s = _shell(0.06, seed=8, protocol=3)
+    x, v = deformed_state(s, seed=81)
+    K = 400
+    with _ctx(s, precision) as a, _ctx(s, precision) as b:
+        b.set_tuning(node_block=128, fused=1)
+        for c in (a, b):
+            c.set_state(0, x, v)
+            c.set_dial_clock(0, 0.0, 1.0, K, 100, True)
+        ra = a.step(37, equil_every=9, force_tol=1e-30, speed_tol=1e-30)
+        rb = b.step(37, equil_every=9, force_tol=1e-30, speed_tol=1e-30)
+        assert a.dial_factor(0) == b.dial_factor(0) == pytest.approx((100 + 37) / K)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+        assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
