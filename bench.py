@@ -112,19 +112,20 @@ def step_bytes(T: int, N: int, mixed: bool, uniform: bool) -> int:
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_grf_disc_1.8M.txt")
+BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_ragged_batch_512.txt")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), or
     None when the summary is absent."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(profile) as f:
             for line in f:
                 if line.startswith("TRAFFIC_JSON "):
                     d = json.loads(line[len("TRAFFIC_JSON "):])
                     if kernel in d:
-                        return dict(d[kernel], source=os.path.relpath(TRAFFIC_PROFILE, ROOT))
+                        return dict(d[kernel], source=os.path.relpath(profile, ROOT))
     except OSError:
         pass
     return None
@@ -312,8 +313,21 @@ def bench_single(args) -> dict:
 
 # ------------------------------------------------------------------- batch
 
+def batch_traffic(tuned: dict, kernel_ms: float):
+    """The fused batch kernel's HBM bytes per launch from the committed
+    counter summary (same batch and tuning) over this run's kernel time."""
+    if not tuned.get("fused") or tuned.get("node_block") != 256:
+        return None
+    t = measured_traffic("tile_step_kernel<true, false, false, true>", BATCH_TRAFFIC_PROFILE)
+    if not t:
+        return None
+    gbs = (t["fetch_bytes"] + t["write_bytes"]) / (kernel_ms * 1e-3) / 1e9
+    return {"bytes_per_launch": t["fetch_bytes"] + t["write_bytes"], "GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+            "source": t["source"]}
+
+
 def bench_batch(args) -> dict:
-    from morphoshell_amd import autotune, workloads
+    from morphoshell_amd import autotune, native, workloads
     from morphoshell_amd.shell import Context
 
     t0 = time.time()
@@ -327,8 +341,15 @@ def bench_batch(args) -> dict:
     with Context(shells, precision="mixed") as ctx:
         for i, (x, v) in enumerate(states):
             ctx.set_state(i, x, v)
-        tuned = autotune.tune(ctx, steps=20, use_cache=False, verbose=args.verbose)
-        log(f"autotuned: {tuned}")
+        if args.no_autotune:
+            # the shape the autotuner picks for this batch
+            tuned = {"tri_block": 256, "node_block": 256, "tile_chunks": 1, "grid_per_cu": 4, "graph_steps": 16,
+                     "fused": 1, "cached": None}
+            ctx.set_tuning(tri_block=256, node_block=256, tri_grid_per_cu=4, node_grid_per_cu=4, graph_steps=16,
+                           fused=1, tile_chunks=1)
+        else:
+            tuned = autotune.tune(ctx, steps=20, use_cache=False, verbose=args.verbose)
+        log(f"tuning: {tuned}")
         secs = time_steps(ctx, states, args.steps, args.warmup, strict=False)
         reps = ctx.step(0, raise_on_blowup=False)
         blown = [i for i, r in enumerate(reps) if r["status"] != 0]
@@ -340,7 +361,8 @@ def bench_batch(args) -> dict:
             ctx.relax_setup(i, steps_per_dial_phase=1000, dial_spacing=1.0, dial_start=0.0, equil_interval=200,
                             tol_f=1e-6, tol_v=1e-5)
         t1 = time.perf_counter()
-        reps = ctx.relax_run(args.relax_steps, poll_every=2000)
+        reps = ctx.relax_run(args.relax_steps, poll_every=2000) if args.relax_steps > 0 else \
+            [{"status": 0, "failed": 0, "max_force_ratio": float("nan")} for _ in shells]
         t_relax = time.perf_counter() - t1
     done = sum(1 for r in reps if r["status"] == 2)
     failed = sum(1 for r in reps if r["failed"])
@@ -348,7 +370,15 @@ def bench_batch(args) -> dict:
     fr = np.array([r["max_force_ratio"] for r in reps if not r["failed"]], float)
     at_tol = {f"{t:g}": int(np.sum(fr <= t)) for t in (1e-3, 1e-4, 1e-5, 1e-6)}
     kb = kernel_bytes(T, N, True, uni)
+    cal = None
+    if args.calibrate:
+        cal_ms = np.zeros(1)
+        native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
+        cal = {"copy_kernel_GBs": 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9}
+        native.check(native.lib().ms_calibration_read(1 << 30, 5, native.dptr(cal_ms)))
+        cal["read_stream_GBs"] = (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
     return {
+        "calibration": cal,
         "metric": "triangle force-evals/s (ragged batch)",
         "value": T_live * args.steps / secs,
         "shells_blown_up_in_timing": len(blown),
@@ -364,6 +394,7 @@ def bench_batch(args) -> dict:
         "kernel_ms": ({"tile_step": kt[0], "bookkeeping": kt[1]} if tuned.get("fused") else
                       {"tri": kt[0], "node": kt[1]}),
         "dominant_alg_GBs": (step_bytes(T, N, True, uni) if tuned.get("fused") else kb[0]) / (kt[0] * 1e-3) / 1e9,
+        "dominant_measured_traffic": batch_traffic(tuned, kt[0]),
         "step_alg_GBs": step_bytes(T, N, True, uni) / (secs / args.steps) / 1e9,
         "tri_record_bytes": tri_bytes(True, uni),
         "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
@@ -686,6 +717,7 @@ def main() -> int:
     ap.add_argument("--precisions", default="fp64,mixed", help="single workload: precisions to time")
     ap.add_argument("--forms", default="fused,two_launch", help="single workload: step forms to time")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batch workload: the autotuner's usual pick, no search")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only (a shell, or a batch of shells, lives on one device); "
