@@ -406,6 +406,127 @@ def bench_batch(args) -> dict:
     }
 
 
+def bench_batch_relax(args) -> dict:
+    """Time to equilibrium of a whole ragged batch, plain FP64 against mixed
+    precision finished in FP64.  Shells as in the batch workload (discs and
+    squares, per-shell random-field lambda, bbar = kappa I, h in [0.002,
+    0.02]) but with triangle counts log-uniform in [batch_tmin, batch_tmax]:
+    at [2k, 100k] a shell needs millions of steps.  Every shell is dialled
+    over 1000 steps and checked every 200 (main.cpp:269-507 per shell); the
+    mixed phase stops each shell at a residual of mixed_tol F_char (its
+    forces carry an FP32 error of a few 1e-6 F_char), then the context
+    switches to FP64 and every shell that did not fail is re-armed, fully
+    dialled, to the real tolerances."""
+    from morphoshell_amd import autotune, workloads
+    from morphoshell_amd.shell import Context
+    from oracle import oracle as orc
+
+    t0 = time.time()
+    shells = workloads.ragged_batch(args.shells, tmin=args.batch_tmin, tmax=args.batch_tmax, precision="mixed")
+    t_setup = time.time() - t0
+    T = sum(s.T for s in shells)
+    N = sum(s.N for s in shells)
+    states = [workloads.flat_noisy_state(s, seed=i) for i, s in enumerate(shells)]
+    tol_f, tol_v, mixed_tol = 1e-6, 1e-5, 1e-5
+    K, interval, chunk, cap = 1000, 200, 50_000, args.batch_relax_max_steps
+    log(f"batch relax: {len(shells)} shells, T={T} N={N}, set-up {t_setup:.1f}s")
+
+    def arm(ctx, tf, tv, start=0.0, k=K, only=None):
+        for i, (x, v) in enumerate(states):
+            if only is not None and i not in only:
+                continue
+            if start == 0.0:
+                ctx.set_state(i, x, v)
+            ctx.relax_setup(i, steps_per_dial_phase=k, dial_spacing=1.0, dial_start=start, equil_interval=interval,
+                            tol_f=tf, tol_v=tv)
+
+    def run(ctx, tag):
+        steps, t1 = 0, time.perf_counter()
+        while steps < cap:
+            reps = ctx.relax_run(chunk, poll_every=10_000)
+            steps += chunk
+            done = sum(1 for r in reps if r["status"] == 2)
+            failed = sum(1 for r in reps if r["failed"])
+            log(f"{tag}: {steps} steps, {time.perf_counter() - t1:.1f}s, {done} at equilibrium, {failed} failed")
+            if all(r["halted"] for r in reps):
+                break
+        return reps, time.perf_counter() - t1
+
+    def energies(ctx):
+        return np.array([sum(ctx.calc_energies(i)[:2]) for i in range(len(shells))])
+
+    out = {"metric": "time-to-equilibrium (ragged batch)", "unit": "s", "higher_is_better": False,
+           "config": {"workload": f"ragged batch of {len(shells)} discs/squares, triangles log-uniform in "
+                                  f"[{args.batch_tmin}, {args.batch_tmax}], per-shell GRF lambda, bbar = kappa I, "
+                                  f"h in [0.002, 0.02]; dialled over {K} steps, checks every {interval}; "
+                                  f"force tol {tol_f}, speed tol {tol_v}",
+                      "num_shells": len(shells), "num_tris": T, "num_nodes": N}}
+    # plain FP64
+    with Context(shells, precision="fp64") as ctx:
+        arm(ctx, tol_f, tol_v)
+        tuned64 = autotune.tune(ctx, steps=20, use_cache=False)
+        arm(ctx, tol_f, tol_v)
+        r64, t64 = run(ctx, "fp64")
+        E64 = energies(ctx)
+    ok64 = [i for i, r in enumerate(r64) if r["status"] == 2]
+    n64 = np.array([(r["eq_step"] + 1) if r["status"] == 2 else r["steps"] for r in r64], float)
+    # mixed, finished in FP64
+    with Context(shells, precision="mixed") as ctx:
+        arm(ctx, mixed_tol, mixed_tol)
+        tunedm = autotune.tune(ctx, steps=20, use_cache=False)
+        arm(ctx, mixed_tol, mixed_tol)
+        rm, tm1 = run(ctx, "mixed")
+        live = {i for i, r in enumerate(rm) if not r["failed"]}
+        ctx.set_precision("fp64")
+        arm(ctx, tol_f, tol_v, start=1.0, k=1, only=live)
+        rf, tm2 = run(ctx, "mixed, fp64 finish")
+        Em = energies(ctx)
+    both = [i for i in ok64 if rf[i]["status"] == 2 and i in live]
+    rel = np.abs(Em[both] - E64[both]) / np.abs(E64[both]) if both else np.zeros(0)
+    tm = tm1 + tm2
+    out["fp64"] = {"seconds": t64, "shells_at_equilibrium": len(ok64),
+                   "shells_failed": sum(1 for r in r64 if r["failed"]),
+                   "shells_per_s": len(ok64) / t64 if t64 > 0 else 0.0,
+                   "steps_to_equilibrium": {"median": float(np.median(n64)), "max": float(n64.max())}}
+    out["mixed_then_fp64"] = {"seconds": tm, "seconds_mixed": tm1, "seconds_fp64_finish": tm2,
+                              "shells_at_equilibrium": sum(1 for r in rf if r["status"] == 2),
+                              "shells_failed": len(shells) - len(live) + sum(1 for i in live if rf[i]["failed"]),
+                              "shells_per_s": sum(1 for r in rf if r["status"] == 2) / tm if tm > 0 else 0.0,
+                              "energy_rel_diff_vs_fp64": {
+                                  "shells_compared": len(both),
+                                  "median": float(np.median(rel)) if rel.size else None,
+                                  "max": float(rel.max()) if rel.size else None,
+                                  "within_1e-6": int(np.sum(rel <= 1e-6))}}
+    out["speedup_mixed_vs_fp64"] = t64 / tm if tm > 0 else None
+    out["tuning"] = {"fp64": tuned64, "mixed": tunedm}
+    if not args.no_cpu:
+        # the oracle's per-step time of the protocol on a few shells spanning
+        # the sizes, fitted as a + b T and summed over every shell's FP64
+        # step count
+        order = np.argsort([s.T for s in shells])
+        pick = [int(order[int(q * (len(shells) - 1))]) for q in (0.0, 0.25, 0.5, 0.75, 1.0)]
+        steps_cpu, ts, tt = 200, [], []
+        for i in pick[:1] + pick:  # the first run warms the thread pool and is not kept
+            sh = shells[i]
+            sysm = orc.System.from_shell(sh)
+            xx, vv = states[i][0].copy(), states[i][1].copy()
+            eta = sh.scales["damping_factor"]
+            tc = time.perf_counter()
+            orc.run_protocol(sysm, lambda f, sh=sh: orc.dial(sh.T, 1, False, f, sh.mesh.abar_info, sh.mesh.bbar_info),
+                             xx, vv, K, 1.0, 0.0, interval, eta, eta, tol_f, tol_v, steps_cpu)
+            ts.append((time.perf_counter() - tc) / steps_cpu)
+            tt.append(sh.T)
+        ts, tt = ts[1:], tt[1:]
+        b, a = np.polyfit(np.array(tt, float), np.array(ts, float), 1)
+        per_shell = np.maximum(a + b * np.array([s.T for s in shells], float), min(ts))
+        out["cpu_baseline"] = {"seconds_extrapolated": float(np.sum(per_shell * n64)), "cores": orc.lib().orc_num_threads(),
+                               "kind": "port",
+                               "sample": f"{steps_cpu} protocol steps on each of {len(pick)} shells "
+                                         f"(T = {', '.join(str(t) for t in tt)}), per-step time fitted as a + b T "
+                                         f"and summed over every shell's FP64 step count"}
+    return out
+
+
 # ------------------------------------------------------------------- relax
 
 def bench_relax_large(args) -> dict:
@@ -703,11 +824,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large", "c5"])
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax"])
     ap.add_argument("--spacing", type=float, default=0.002)
     ap.add_argument("--shells", type=int, default=512)
     ap.add_argument("--relax-steps", type=int, default=20000)
     ap.add_argument("--relax-large-steps", type=int, default=60000)
+    ap.add_argument("--batch-tmin", type=int, default=500, help="batch_relax workload: smallest shell (triangles)")
+    ap.add_argument("--batch-tmax", type=int, default=4000, help="batch_relax workload: largest shell (triangles)")
+    ap.add_argument("--batch-relax-max-steps", type=int, default=4_000_000,
+                    help="batch_relax workload: step bound per phase")
     ap.add_argument("--c5-spacing", type=float, default=0.001)
     ap.add_argument("--c5-steps", type=int, default=60000)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
@@ -724,7 +849,7 @@ def main() -> int:
               "runs on several GPUs are not part of it.", file=sys.stderr)
         return 2
     fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax,
-          "relax_large": bench_relax_large, "c5": bench_c5}[args.workload]
+          "relax_large": bench_relax_large, "c5": bench_c5, "batch_relax": bench_batch_relax}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -52,6 +52,17 @@ struct ms_ctx {
     Chunk* d_node_chunks = nullptr;
     int n_tri_chunks = 0, n_node_chunks = 0;
     size_t part_len = 0;
+    std::vector<Chunk> h_tri_chunks, h_node_chunks;  // host copies of the full tables
+
+    // Step launches over the running shells only (see refresh_active): the
+    // running shells' chunks and tiles, rebuilt when the set of halted shells
+    // changes; act_all: every shell runs and the full tables are used.
+    std::vector<char> act_live;      // running flags the tables were built for (empty: stale)
+    bool act_all = true;
+    Chunk* d_act_tri = nullptr;
+    Chunk* d_act_node = nullptr;
+    int* d_act_tiles = nullptr;
+    int n_act_tri = 0, n_act_node = 0, n_act_tiles = 0;
 
     // fused tile step (built on demand for the current node chunks)
     TileData tile{};                 // device metadata + tile-ordered records
@@ -111,27 +122,35 @@ int build_chunks(ms_ctx* c) {
         ShellDev& S = c->hs[s];
         S.tri_chunk_begin = (int)tc.size();
         for (int t = S.tri_begin; t < S.tri_begin + S.num_tris; t += tb)
-            tc.push_back(make_int4(t, std::min(t + tb, S.tri_begin + S.num_tris), s, 0));
+            tc.push_back(make_int4(t, std::min(t + tb, S.tri_begin + S.num_tris), s, (int)tc.size()));
         S.tri_chunk_count = (int)tc.size() - S.tri_chunk_begin;
         S.node_chunk_begin = (int)nc.size();
         for (int n = S.node_begin; n < S.node_begin + S.num_nodes; n += nb)
-            nc.push_back(make_int4(n, std::min(n + nb, S.node_begin + S.num_nodes), s, 0));
+            nc.push_back(make_int4(n, std::min(n + nb, S.node_begin + S.num_nodes), s, (int)nc.size()));
         S.node_chunk_count = (int)nc.size() - S.node_chunk_begin;
     }
     if (c->d_tri_chunks) hipFree(c->d_tri_chunks);
     if (c->d_node_chunks) hipFree(c->d_node_chunks);
     if (c->b.part) hipFree(c->b.part);
     if (c->b.part_i) hipFree(c->b.part_i);
-    c->d_tri_chunks = c->d_node_chunks = nullptr;
+    if (c->d_act_tri) hipFree(c->d_act_tri);
+    if (c->d_act_node) hipFree(c->d_act_node);
+    c->d_tri_chunks = c->d_node_chunks = c->d_act_tri = c->d_act_node = nullptr;
     c->b.part = nullptr;
     c->b.part_i = nullptr;
+    c->act_live.clear();
+    c->act_all = true;
     HIPC(c, hipMalloc((void**)&c->d_tri_chunks, tc.size() * sizeof(Chunk)));
     HIPC(c, hipMalloc((void**)&c->d_node_chunks, nc.size() * sizeof(Chunk)));
+    HIPC(c, hipMalloc((void**)&c->d_act_tri, tc.size() * sizeof(Chunk)));
+    HIPC(c, hipMalloc((void**)&c->d_act_node, nc.size() * sizeof(Chunk)));
     HIPC(c, hipMemcpy(c->d_tri_chunks, tc.data(), tc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->d_node_chunks, nc.data(), nc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
     c->n_tri_chunks = (int)tc.size();
     c->n_node_chunks = (int)nc.size();
     c->part_len = 4 * (size_t)std::max(tc.size(), nc.size());
+    c->h_tri_chunks = std::move(tc);
+    c->h_node_chunks = std::move(nc);
     HIPC(c, hipMalloc((void**)&c->b.part, c->part_len * sizeof(double)));
     HIPC(c, hipMalloc((void**)&c->b.part_i, c->part_len * sizeof(int)));
     c->tiles_ready = false;
@@ -156,6 +175,8 @@ void free_tiles(ms_ctx* c) {
     c->dev_bytes -= c->tile_bytes;
     c->tile_bytes = 0;
     c->tiles_ready = false;
+    c->d_act_tiles = nullptr;  // one of the metadata allocations
+    c->act_live.clear();
 }
 
 // Tile-ordered copies of the per-triangle streams the mechanics reads, in
@@ -348,10 +369,11 @@ int build_tiles(ms_ctx* c) {
         *dptr = static_cast<typename std::remove_pointer<decltype(dptr)>::type>(p);
         return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice);
     };
-    int *d_toff, *d_tend, *d_tris, *d_hoff, *d_halo;
+    int *d_toff, *d_tend, *d_tris, *d_hoff, *d_halo, *d_act;
     int4* d_tiles;
     unsigned short *d_lpid, *d_dst, *d_poff;
     HIPC(c, upload(tiles, &d_tiles));
+    HIPC(c, upload(std::vector<int>(tiles.size(), 0), &d_act));  // room for the running tiles' list
     HIPC(c, upload(toff, &d_toff));
     HIPC(c, upload(tend, &d_tend));
     HIPC(c, upload(tris, &d_tris));
@@ -370,6 +392,10 @@ int build_tiles(ms_ctx* c) {
     td.lpid = d_lpid;
     td.dst = d_dst;
     td.poff = d_poff;
+    td.active = nullptr;
+    td.nactive = 0;
+    c->d_act_tiles = d_act;
+    c->act_live.clear();
     c->tile_T = (int)tris.size();
     if (int r = build_tile_records(c)) return r;
     c->tiles_ready = true;
@@ -445,7 +471,60 @@ void params_to_dev(const ms_shell_params& p, ShellDev& S) {
     S.from_ansatz = p.dial_from_ansatz ? 1 : 0;
 }
 
-TileData tile_data(const ms_ctx* c) { return c->tile; }
+void destroy_graph(ms_ctx* c);
+
+TileData tile_data(const ms_ctx* c) {
+    TileData t = c->tile;
+    if (!c->act_all) {
+        t.active = c->d_act_tiles;
+        t.nactive = c->n_act_tiles;
+    }
+    return t;
+}
+
+// The chunk and tile tables of the step launches: every shell's, or (after
+// some shells halted) the running shells' only.  Halted shells are skipped
+// inside the step kernels either way, so this changes which workgroups are
+// launched, not the results; per-chunk partials keep their full-table index
+// (Chunk .w, tile .w).  Rebuilt when the running set differs from the one
+// the tables were built for; the captured graphs hold the old tables and are
+// destroyed.
+int refresh_active(ms_ctx* c) {
+    std::vector<char> live(c->num_shells);
+    bool all = true;
+    for (int s = 0; s < c->num_shells; ++s) {
+        live[s] = c->hs[s].halted ? 0 : 1;
+        all = all && live[s];
+    }
+    if (live == c->act_live) return MS_OK;
+    destroy_graph(c);
+    c->act_live = live;
+    c->act_all = all;
+    if (all) return MS_OK;
+    std::vector<Chunk> tc, nc;
+    for (const Chunk& ch : c->h_tri_chunks)
+        if (live[ch.z]) tc.push_back(ch);
+    for (const Chunk& ch : c->h_node_chunks)
+        if (live[ch.z]) nc.push_back(ch);
+    if (!tc.empty())
+        HIPC(c, hipMemcpy(c->d_act_tri, tc.data(), tc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    if (!nc.empty())
+        HIPC(c, hipMemcpy(c->d_act_node, nc.data(), nc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    c->n_act_tri = (int)tc.size();
+    c->n_act_node = (int)nc.size();
+    c->n_act_tiles = 0;
+    if (c->tiles_ready) {
+        std::vector<int4> tiles(c->tile.ntiles);
+        HIPC(c, hipMemcpy(tiles.data(), c->tile.tiles, tiles.size() * sizeof(int4), hipMemcpyDeviceToHost));
+        std::vector<int> at;
+        for (int i = 0; i < (int)tiles.size(); ++i)
+            if (live[tiles[i].z]) at.push_back(i);
+        if (!at.empty())
+            HIPC(c, hipMemcpy(c->d_act_tiles, at.data(), at.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->n_act_tiles = (int)at.size();
+    }
+    return MS_OK;
+}
 
 // Some shell uses the LCE dial protocol (its dial info has the 5 LCE columns).
 bool has_lce(const ms_ctx* c) { return c->acols == 5; }
@@ -463,17 +542,23 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
-        const Launch tl{grid_for(c, c->tile.ntiles, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
-        launch_tile_step(tl, nd, c->b, tile_data(c), c->d_shells, mixed, has_lce(c), uni_data(c), fl);
+        const TileData tld = tile_data(c);
+        const int nt = tld.active ? tld.nactive : tld.ntiles;
+        const Launch tl{grid_for(c, nt, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
+        launch_tile_step(tl, nd, c->b, tld, c->d_shells, mixed, has_lce(c), uni_data(c), fl);
         if (timed) hipEventRecord(c->ev[1], c->stream);
         launch_step_bookkeeping(c->stream, nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
                                 c->d_clock, fl);
     } else {
-        launch_tri_forces(tri_launch(c), td, c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, mixed, has_lce(c),
-                          uni_data(c), true);
+        const Chunk* tcs = c->act_all ? c->d_tri_chunks : c->d_act_tri;
+        const Chunk* ncs = c->act_all ? c->d_node_chunks : c->d_act_node;
+        const int ntc = c->act_all ? c->n_tri_chunks : c->n_act_tri;
+        const int nnc = c->act_all ? c->n_node_chunks : c->n_act_node;
+        const Launch tl{grid_for(c, ntc, c->tune.tri_grid_per_cu), c->tune.tri_block, c->stream};
+        const Launch nl{grid_for(c, nnc, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
+        launch_tri_forces(tl, td, c->b, tcs, ntc, c->d_shells, mixed, has_lce(c), uni_data(c), true);
         if (timed) hipEventRecord(c->ev[1], c->stream);
-        launch_node_step(node_launch(c), nd, c->b, c->d_node_chunks, c->n_node_chunks, c->d_shells, c->num_shells,
-                         c->d_clock, mixed, fl);
+        launch_node_step(nl, nd, c->b, ncs, nnc, c->d_shells, c->num_shells, c->d_clock, mixed, fl);
     }
     if (timed) hipEventRecord(c->ev[2], c->stream);
 }
@@ -746,6 +831,8 @@ extern "C" void ms_destroy(ms_ctx* c) {
     if (c->d_node_chunks) hipFree(c->d_node_chunks);
     if (c->b.part) hipFree(c->b.part);
     if (c->b.part_i) hipFree(c->b.part_i);
+    if (c->d_act_tri) hipFree(c->d_act_tri);
+    if (c->d_act_node) hipFree(c->d_act_node);
     if (c->scratch3) hipFree(c->scratch3);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
@@ -1085,6 +1172,7 @@ extern "C" int ms_step(ms_ctx* c, int64_t nsteps, int64_t equil_every, double to
         c->hs[s].checked = 0;
     }
     if (int r = push_shells(c)) return r;
+    if (int r = refresh_active(c)) return r;
     const bool use_graph = !c->timing;
     if (use_graph)
         if (int r = ensure_graph(c)) return r;

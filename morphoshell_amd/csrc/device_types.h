@@ -85,15 +85,15 @@ __host__ __device__ inline double shell_dial_factor(const ShellDev& s) {
 }
 
 // A contiguous range of triangles or nodes of one shell, processed by one
-// block: {begin, end, shell, unused}.
+// block: {begin, end, shell, index in the context's full chunk table}.  A
+// step launch may take a table of the running shells' chunks only; the index
+// keeps per-chunk partial results at their place in the full table.
 typedef int4 Chunk;
 
-// Global clock of a context: steps taken and the completion counter used by
-// the last-block bookkeeping of node launches.
+// Global clock of a context: steps taken.
 struct ClockDev {
     long long steps;
-    unsigned int done;
-    unsigned int pad_;
+    long long pad_;
 };
 
 }  // namespace ms

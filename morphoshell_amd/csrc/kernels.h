@@ -162,6 +162,8 @@ struct TileData {
     int max_entries;               // max over tiles and passes of pass entries
     const double* rec64;           // REC_SLOTS * 64 doubles per block of 64 entries
     const float* rec32;            // REC_SLOTS * 64 floats per block
+    const int* active;             // null: every tile; else the tiles to run (nactive)
+    int nactive;
 };
 // Record gather: component k < ncomp of stream `in` (SoA, [k*T + t]) into
 // slot `slot + k` of every entry j < Tp with tris[j] >= 0 (elem_bytes 4 or 8).

@@ -306,11 +306,11 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
             double v2[2] = {fnorm, vnorm};
             int i2[2] = {fidx, vidx};
             block_argmax<2>(v2, i2);
-            if (threadIdx.x == 0) {
-                k.b.part[2 * ci] = v2[0];
-                k.b.part[2 * ci + 1] = v2[1];
-                k.b.part_i[2 * ci] = i2[0];
-                k.b.part_i[2 * ci + 1] = i2[1];
+            if (threadIdx.x == 0) {  // at the chunk's place in the full table
+                k.b.part[2 * ch.w] = v2[0];
+                k.b.part[2 * ch.w + 1] = v2[1];
+                k.b.part_i[2 * ch.w] = i2[0];
+                k.b.part_i[2 * ch.w + 1] = i2[1];
             }
         }
     }

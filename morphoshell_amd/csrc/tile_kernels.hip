@@ -96,8 +96,10 @@ void tile_step_kernel(TileKArgs k) {
     double* lpos = smem;
     double* lsum = smem + 3 * (size_t)k.tl.max_nodes;
     CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)stride : lsum);
-    for (int vi = blockIdx.x; vi < k.tl.ntiles; vi += gridDim.x) {
-        const int ci = xcd_remap(vi, k.tl.ntiles);
+    const int nt = k.tl.active ? k.tl.nactive : k.tl.ntiles;
+    for (int vi = blockIdx.x; vi < nt; vi += gridDim.x) {
+        const int vr = xcd_remap(vi, nt);
+        const int ci = k.tl.active ? uni(k.tl.active[vr]) : vr;
         // nodes [ch.x, ch.y) of shell ch.z; chunk ch.w + r holds nodes from ch.x + r * B
         const int4 ch = make_int4(uni(k.tl.tiles[ci].x), uni(k.tl.tiles[ci].y), uni(k.tl.tiles[ci].z),
                                   uni(k.tl.tiles[ci].w));

@@ -721,3 +721,45 @@ def test_lce_shell_fused_equals_two_launch_while_dialling(precision):
         assert a.dial_factor(0) == b.dial_factor(0) == pytest.approx((100 + 37) / K)
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
         assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
+
+
+@pytest.mark.parametrize("precision,fused", [("fp64", 0), ("mixed", 0), ("fp64", 1), ("mixed", 1)])
+def test_halted_shells_drop_out_of_the_step_launches(precision, fused):
+    """Once shells halt, the step launches cover the running shells' chunks
+    and tiles only.  The running shells' states and checked residuals equal
+    those of a context where nothing halts, bit for bit, the halted shells'
+    states stay as they halted, and a re-armed shell runs again."""
+    shells = [_shell(0.12 + 0.04 * (i % 3), seed=i, h=0.02, kappa=0.4 - 0.2 * i) for i in range(6)]
+    states = [deformed_state(sh, seed=200 + i) for i, sh in enumerate(shells)]
+    halting = (1, 4)
+    with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        for c in (a, b):
+            c.set_tuning(node_block=64, tri_block=64, fused=fused)
+            for i, st in enumerate(states):
+                c.set_state(i, *st)
+        for i in halting:  # fully dialled, any residual counts as equilibrium: halts at its first check
+            a.relax_setup(i, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=2,
+                          tol_f=1e30, tol_v=1e30)
+        ra = a.step(5)
+        assert all(ra[i]["halted"] for i in halting)
+        held = {i: a.get_state(i) for i in halting}
+        b.step(5)
+        ra = a.step(40, equil_every=6, force_tol=1e-30, speed_tol=1e-30)
+        rb = b.step(40, equil_every=6, force_tol=1e-30, speed_tol=1e-30)
+        for i in range(len(shells)):
+            xa, va = a.get_state(i)
+            if i in halting:
+                assert np.array_equal(xa, held[i][0]) and np.array_equal(va, held[i][1]), i
+                continue
+            xb, vb = b.get_state(i)
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb), i
+            assert ra[i]["max_force_ratio"] == rb[i]["max_force_ratio"], i
+            assert ra[i]["max_force_node"] == rb[i]["max_force_node"], i
+        # re-armed: the shell is in the launches again
+        a.relax_setup(1, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=2,
+                      tol_f=1e-30, tol_v=1e-30)
+        a.step(10)
+        b.step(10)
+        assert not np.array_equal(a.get_state(1)[0], held[1][0])
+        for i in (0, 2, 3, 5):
+            assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
