@@ -499,6 +499,33 @@ def bench_batch_relax(args) -> dict:
                                   "within_1e-6": int(np.sum(rel <= 1e-6))}}
     out["speedup_mixed_vs_fp64"] = t64 / tm if tm > 0 else None
     out["tuning"] = {"fp64": tuned64, "mixed": tunedm}
+    if args.batch_control:
+        # control: FP64 again from the same states with a 1e-10 z-perturbation,
+        # to separate a precision effect from the sensitivity of these
+        # multistable shells to any rounding-level change of the trajectory
+        rng = np.random.default_rng(12345)
+        base = states
+        states = []
+        for sh, (x, v) in zip(shells, base):
+            xp = x.copy()
+            xp[2 * sh.N:] += 1e-10 * rng.standard_normal(sh.N)
+            states.append((xp, v))
+        with Context(shells, precision="fp64") as ctx:
+            ctx.set_tuning(tri_block=tuned64["tri_block"], node_block=tuned64["node_block"],
+                           tri_grid_per_cu=tuned64["grid_per_cu"], node_grid_per_cu=tuned64["grid_per_cu"],
+                           graph_steps=tuned64["graph_steps"], fused=tuned64["fused"], tile_chunks=tuned64["tile_chunks"])
+            arm(ctx, tol_f, tol_v)
+            rc, tc = run(ctx, "fp64 control")
+            Ec = energies(ctx)
+        states = base
+        bc = [i for i in ok64 if rc[i]["status"] == 2]
+        relc = np.abs(Ec[bc] - E64[bc]) / np.abs(E64[bc]) if bc else np.zeros(0)
+        out["control_fp64_perturbed_1e-10"] = {
+            "seconds": tc, "shells_at_equilibrium": sum(1 for r in rc if r["status"] == 2),
+            "energy_rel_diff_vs_fp64": {"shells_compared": len(bc),
+                                        "median": float(np.median(relc)) if relc.size else None,
+                                        "max": float(relc.max()) if relc.size else None,
+                                        "within_1e-6": int(np.sum(relc <= 1e-6))}}
     if not args.no_cpu:
         # the oracle's per-step time of the protocol on a few shells spanning
         # the sizes, fitted as a + b T and summed over every shell's FP64
@@ -833,6 +860,8 @@ def main() -> int:
     ap.add_argument("--batch-tmax", type=int, default=4000, help="batch_relax workload: largest shell (triangles)")
     ap.add_argument("--batch-relax-max-steps", type=int, default=4_000_000,
                     help="batch_relax workload: step bound per phase")
+    ap.add_argument("--batch-control", action="store_true",
+                    help="batch_relax workload: also an FP64 run from states perturbed by 1e-10")
     ap.add_argument("--c5-spacing", type=float, default=0.001)
     ap.add_argument("--c5-steps", type=int, default=60000)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
