@@ -18,12 +18,16 @@ The CPU baseline is the FP64 OpenMP oracle on the host cores, on a bounded
 number of steps of the same shell.
 
 Other workloads (one JSON line each):
-  --workload batch   ragged batch of 512 discs/squares, 2k-100k triangles,
-                     mixed mode, autotuned launch shapes
-  --workload relax   time-to-equilibrium of the small defect disc: FP64 vs
-                     mixed-then-FP64, beside the oracle on the host
+  --workload batch        ragged batch of 512 discs/squares, 2k-100k triangles,
+                          mixed mode, autotuned launch shapes
+  --workload relax        time-to-equilibrium of the small defect disc: FP64 vs
+                          mixed-then-FP64, beside the oracle on the host
+  --workload relax_large  the large shell under the device-run protocol, bounded
+  --workload c5           the thin-sheet stress test (warm start, then mixed), bounded
+  --workload batch_relax  a 512-shell batch of 500-4000 triangles relaxed to
+                          equilibrium, FP64 vs mixed-then-FP64
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload single|batch|relax]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 from __future__ import annotations
 
@@ -131,26 +135,69 @@ def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
     return None
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sh, x, v, budget_s: float) -> dict:
+    """The FP64 oracle's step throughput on the host: the median of up to 5
+    runs of 100 steps with every OpenMP thread (fewer runs if the budget is
+    short), and a shorter run on one thread."""
     from oracle import oracle as orc
 
-    threads = orc.lib().orc_num_threads()
+    L = orc.lib()
+    threads = L.orc_num_threads()
     sysm = orc.System.from_shell(sh)
     xx, vv = x.copy(), v.copy()
+
+    def steps(n):
+        if sysm.run_steps(xx, vv, n) != -1:
+            raise RuntimeError("CPU baseline: the oracle's step hit the blow-up test; the timing would be void")
+
     t0 = time.perf_counter()
-    sysm.run_steps(xx, vv, 1)
-    t1 = time.perf_counter() - t0
-    n = int(max(2, min(200, budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    sysm.run_steps(xx, vv, n)
-    dt = time.perf_counter() - t0
+    steps(2)  # warm-up
+    t1 = (time.perf_counter() - t0) / 2
+    n = 100
+    runs = int(max(1, min(5, 0.8 * budget_s / (n * t1))))
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        steps(n)
+        rates.append(sh.T * n / (time.perf_counter() - t0))
+    L.orc_set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        steps(1)
+        t_one = time.perf_counter() - t0
+        n1 = int(max(2, min(20, 0.2 * budget_s / t_one)))
+        t0 = time.perf_counter()
+        steps(n1)
+        one = sh.T * n1 / (time.perf_counter() - t0)
+    finally:
+        L.orc_set_num_threads(threads)
+    try:
+        logical = len(os.sched_getaffinity(0))
+    except AttributeError:
+        logical = os.cpu_count()
     return {
-        "value": sh.T * n / dt,
+        "value": float(np.median(rates)),
         "unit": "triangle force-evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} FP64 steps of the same {sh.T}-triangle shell (q=M x, dW/dq, F=-M^T g, damping, "
-                  f"clamp, update) with {threads} OpenMP threads",
+        "sample": f"median of {runs} runs of {n} FP64 steps of the same {sh.T}-triangle shell (q=M x, dW/dq, "
+                  f"F=-M^T g, damping, clamp, update) with {threads} OpenMP threads (not pinned: binding "
+                  f"would also pin the process's own threads and the meshing workers it forks)",
+        "runs": [float(r) for r in rates],
+        "one_thread": {"value": one, "steps": n1},
+        "cpu_model": cpu_model(),
+        "logical_cpus_available": logical,
     }
 
 
@@ -864,7 +911,7 @@ def main() -> int:
                     help="batch_relax workload: also an FP64 run from states perturbed by 1e-10")
     ap.add_argument("--c5-spacing", type=float, default=0.001)
     ap.add_argument("--c5-steps", type=int, default=60000)
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="also run the copy kernel used to scale counters")
