@@ -135,6 +135,21 @@ def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
     return None
 
 
+def device_info() -> dict:
+    """The device as the HIP runtime reports it.  The roofline's peak is the
+    published HBM figure: the memory clock in the device properties does not
+    give HBM3E's data rate."""
+    import ctypes as C
+
+    from morphoshell_amd import native
+
+    name = C.create_string_buffer(256)
+    cus, hbm, clk, bus = C.c_int32(0), C.c_int64(0), C.c_int32(0), C.c_int32(0)
+    native.check(native.lib().ms_device_info(name, 256, C.byref(cus), C.byref(hbm), C.byref(clk), C.byref(bus)))
+    return {"name": name.value.decode(errors="replace"), "compute_units": cus.value, "hbm_bytes": hbm.value,
+            "mem_clock_khz": clk.value, "mem_bus_bits": bus.value, "hbm_peak_GBs_published": HBM_PEAK_GBS}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -261,13 +276,16 @@ def bench_single(args) -> dict:
                 res[(prec, form)] = {"secs": secs, "kt": kt}
                 log(f"{prec} {form}: {args.steps} steps {secs * 1e3:.2f} ms -> "
                     f"{T * args.steps / secs / 1e9:.3f} G tri-evals/s; kernel ms {kt[0]:.4f} + {kt[1]:.4f}")
-    if args.calibrate:
+    device = device_info()
+    if not args.no_calibrate:
         cal_ms = np.zeros(1)
         native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
         copy_gbs = 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
         native.check(native.lib().ms_calibration_read(1 << 30, 5, native.dptr(cal_ms)))
         read_gbs = (1 << 30) / (cal_ms[0] * 1e-3) / 1e9
+        device.update(copy_kernel_GBs=float(copy_gbs), read_stream_GBs=float(read_gbs))
         log(f"copy kernel 1 GiB: {copy_gbs:.0f} GB/s; read-only stream 1 GiB: {read_gbs:.0f} GB/s")
+    log(f"device {device}")
 
     cpu, parity = None, {}
     if not args.no_cpu:
@@ -351,10 +369,8 @@ def bench_single(args) -> dict:
         "parity": parity,
         "cpu_baseline": cpu,
         "setup_s": t_setup,
+        "device": device,
     }
-    if args.calibrate:
-        out["copy_kernel_GBs"] = copy_gbs
-        out["read_stream_GBs"] = read_gbs
     return out
 
 
@@ -914,7 +930,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--calibrate", action="store_true", help="also run the copy kernel used to scale counters")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="batch workload: also run the copy and read kernels (the single workload always does)")
+    ap.add_argument("--no-calibrate", action="store_true",
+                    help="single workload: skip the copy and read kernels that measure the HBM ceiling")
     ap.add_argument("--precisions", default="fp64,mixed", help="single workload: precisions to time")
     ap.add_argument("--forms", default="fused,two_launch", help="single workload: step forms to time")
     ap.add_argument("--verbose", action="store_true")

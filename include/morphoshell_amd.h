@@ -360,6 +360,14 @@ int ms_calibration_copy(int64_t nbytes, int32_t reps, double* out_ms);
  * ceiling of a kernel that mostly reads).  out_ms as above. */
 int ms_calibration_read(int64_t nbytes, int32_t reps, double* out_ms);
 
+/* Measurement aid: the current device as the HIP runtime reports it: name
+ * (its architecture name where the marketing name is empty; NUL-terminated,
+ * at most name_len bytes), compute units, HBM bytes, and the memory clock
+ * (kHz) and bus width (bits) of its properties.  Any output pointer may be
+ * NULL. */
+int ms_device_info(char* name, int32_t name_len, int32_t* num_cus, int64_t* hbm_bytes, int32_t* mem_clock_khz,
+                   int32_t* mem_bus_bits);
+
 /* ====================================================== learned warm start
  * A mesh message-passing network that predicts near-equilibrium node
  * positions from the flat reference mesh and the programmed metric.  It is

@@ -1491,6 +1491,24 @@ extern "C" int ms_calibration_read(int64_t nbytes, int32_t reps, double* out_ms)
     return calibration_stream(nbytes, reps, out_ms, true, "ms_calibration_read");
 }
 
+extern "C" int ms_device_info(char* name, int32_t name_len, int32_t* num_cus, int64_t* hbm_bytes,
+                              int32_t* mem_clock_khz, int32_t* mem_bus_bits) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return fail(nullptr, MS_ERR_NO_DEVICE, fmt("ms_device_info: %s", hipGetErrorString(e)));
+    if (name && name_len > 0) {
+        std::strncpy(name, p.name[0] ? p.name : p.gcnArchName, (size_t)name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (num_cus) *num_cus = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+    if (mem_clock_khz) *mem_clock_khz = p.memoryClockRate;
+    if (mem_bus_bits) *mem_bus_bits = p.memoryBusWidth;
+    return MS_OK;
+}
+
 extern "C" int ms_last_step_timing(const ms_ctx* c, double out[3]) {
     if (!c || !out) return MS_ERR_ARG;
     for (int i = 0; i < 3; ++i) out[i] = c->last_ms[i];

@@ -27,8 +27,8 @@ EXPORTED = [
     "ms_impose_constraints", "ms_advance_dynamics", "ms_is_at_equilibrium", "ms_calc_energies",
     "ms_calc_geometry", "ms_calc_energy_densities", "ms_calc_diagnostics", "ms_get_forces", "ms_set_forces", "ms_step",
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
-    "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_calibration_read", "ms_relax_setup",
-    "ms_relax_run", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
+    "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_calibration_read",
+    "ms_device_info", "ms_relax_setup", "ms_relax_run", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
     "ms_nn_set_params", "ms_nn_set_graph", "ms_nn_forward", "ms_nn_loss_backward", "ms_nn_adam_step",
     "ms_nn_gemm_test",
 ]
@@ -191,6 +191,8 @@ def lib() -> C.CDLL:
         "ms_synchronize": (C.c_int, [P]),
         "ms_calibration_copy": (C.c_int, [C.c_int64, C.c_int32, _dp]),
         "ms_calibration_read": (C.c_int, [C.c_int64, C.c_int32, _dp]),
+        "ms_device_info": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "ms_relax_setup": (C.c_int, [P, C.c_int32, C.POINTER(RelaxParams)]),
         "ms_relax_run": (C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(RelaxReport)]),
         "ms_nn_create": (C.c_int, [C.POINTER(NNArch), C.POINTER(P)]),
