@@ -159,7 +159,8 @@ void tile_step_kernel(TileKArgs k) {
                 unsigned t = (unsigned)j;
                 asm volatile("" : "+v"(t));
                 const unsigned* lw = reinterpret_cast<const unsigned*>(k.tl.lpid);
-                const unsigned w0 = ld(lw, 3 * t), w1 = ld(lw, 3 * t + 1), w2 = ld(lw, 3 * t + 2);
+                const unsigned w0 = ld_stream<!MIXED>(lw, 3 * t), w1 = ld_stream<!MIXED>(lw, 3 * t + 1),
+                               w2 = ld_stream<!MIXED>(lw, 3 * t + 2);
                 lp[0] = w0 & 0xFFFFu;
                 lp[1] = w0 >> 16;
                 lp[2] = w1 & 0xFFFFu;
@@ -172,7 +173,8 @@ void tile_step_kernel(TileKArgs k) {
                 unsigned t = (unsigned)j;
                 asm volatile("" : "+v"(t));
                 const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
-                const unsigned d0 = ld(dw, 3 * t), d1 = ld(dw, 3 * t + 1), d2 = ld(dw, 3 * t + 2);
+                const unsigned d0 = ld_stream<!MIXED>(dw, 3 * t), d1 = ld_stream<!MIXED>(dw, 3 * t + 1),
+                               d2 = ld_stream<!MIXED>(dw, 3 * t + 2);
                 const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
 #pragma unroll
                 for (int s = 0; s < 6; ++s) {

@@ -56,6 +56,19 @@ template <typename E>
 __device__ __forceinline__ E ld(const E* base, unsigned i) {
     return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(E));
 }
+// The same for data a step streams once (records, tile-local ids): a
+// non-temporal load when NT, which leaves the caches to the node positions the
+// neighbouring tiles re-read.  Measured on the 1.8M-triangle shell: the fused
+// FP64 step 7% faster; the mixed step (3.8%) and both two-launch steps slower,
+// so only the fused FP64 path sets it.
+template <bool NT, typename E>
+__device__ __forceinline__ E ld_stream(const E* base, unsigned i) {
+    const E* p = reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(E));
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
 
 // The shell quantities the per-triangle mechanics needs, block-uniform.
 struct TriParams {
@@ -104,17 +117,17 @@ struct DialRec {
 
 // CS: component stride of the record streams when known at compile time (the
 // tile launch's blocked records, 64), else 0 for td.T.
-template <bool LCE, bool UNI, int CS = 0>
+template <bool LCE, bool UNI, int CS = 0, bool NT = false>
 __device__ __forceinline__ void load_dial(const TriData& td, const TriParams& s, unsigned t, DialRec& d) {
     const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) d.ai[k] = ld(td.ainfo + k * T, t);
+    for (int k = 0; k < 3; ++k) d.ai[k] = ld_stream<NT>(td.ainfo + k * T, t);
     if (LCE && s.protocol == 3) {
-        d.ai[3] = ld(td.ainfo + 3 * T, t);
-        d.ai[4] = ld(td.ainfo + 4 * T, t);
+        d.ai[3] = ld_stream<NT>(td.ainfo + 3 * T, t);
+        d.ai[4] = ld_stream<NT>(td.ainfo + 4 * T, t);
     } else {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) d.bi[k] = UNI ? s.b[k] : ld(td.binfo + k * T, t);
+        for (int k = 0; k < 3; ++k) d.bi[k] = UNI ? s.b[k] : ld_stream<NT>(td.binfo + k * T, t);
     }
     d.A0[0] = 1.0;
     d.A0[1] = 0.0;
@@ -122,11 +135,11 @@ __device__ __forceinline__ void load_dial(const TriData& td, const TriParams& s,
     d.B0[0] = d.B0[1] = d.B0[2] = 0.0;
     if (s.has_a0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) d.A0[k] = ld(td.a0 + k * T, t);
+        for (int k = 0; k < 3; ++k) d.A0[k] = ld_stream<NT>(td.a0 + k * T, t);
     }
     if (s.has_b0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) d.B0[k] = ld(td.b0 + k * T, t);
+        for (int k = 0; k < 3; ++k) d.B0[k] = ld_stream<NT>(td.b0 + k * T, t);
     }
 }
 
@@ -247,13 +260,13 @@ template <bool LCE, bool UNI, int CS = 0>
 __device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams& sp, unsigned t, RecF64& r) {
     const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) r.gc[k] = ld(td.G + k * T, t);
+    for (int k = 0; k < 6; ++k) r.gc[k] = ld_stream<CS != 0>(td.G + k * T, t);
 #pragma unroll
-    for (int k = 0; k < 18; ++k) r.hc[k] = ld(td.H + k * T, t);
-    r.h = UNI ? sp.h : ld(td.h, t);
-    r.mu = UNI ? sp.mu : ld(td.mu, t);
-    r.A = ld(td.area, t);
-    load_dial<LCE, UNI, CS>(td, sp, t, r.dr);
+    for (int k = 0; k < 18; ++k) r.hc[k] = ld_stream<CS != 0>(td.H + k * T, t);
+    r.h = UNI ? sp.h : ld_stream<CS != 0>(td.h, t);
+    r.mu = UNI ? sp.mu : ld_stream<CS != 0>(td.mu, t);
+    r.A = ld_stream<CS != 0>(td.area, t);
+    load_dial<LCE, UNI, CS, CS != 0>(td, sp, t, r.dr);  // tile records (CS = 64): non-temporal
 }
 
 // FP64 force contributions (out[3*slot + c]) of the triangle whose record is
