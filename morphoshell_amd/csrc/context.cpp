@@ -498,9 +498,14 @@ int refresh_active(ms_ctx* c) {
     }
     if (live == c->act_live) return MS_OK;
     destroy_graph(c);
-    c->act_live = live;
-    c->act_all = all;
-    if (all) return MS_OK;
+    // the full tables until the new ones are in place (an upload error leaves
+    // them in use and the running set unrecorded, so the next call retries)
+    c->act_live.clear();
+    c->act_all = true;
+    if (all) {
+        c->act_live = live;
+        return MS_OK;
+    }
     std::vector<Chunk> tc, nc;
     for (const Chunk& ch : c->h_tri_chunks)
         if (live[ch.z]) tc.push_back(ch);
@@ -523,6 +528,8 @@ int refresh_active(ms_ctx* c) {
             HIPC(c, hipMemcpy(c->d_act_tiles, at.data(), at.size() * sizeof(int), hipMemcpyHostToDevice));
         c->n_act_tiles = (int)at.size();
     }
+    c->act_live = live;
+    c->act_all = false;
     return MS_OK;
 }
 
