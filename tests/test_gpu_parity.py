@@ -763,3 +763,19 @@ def test_halted_shells_drop_out_of_the_step_launches(precision, fused):
         assert not np.array_equal(a.get_state(1)[0], held[1][0])
         for i in (0, 2, 3, 5):
             assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
+
+
+def test_device_info_reports_the_device():
+    import ctypes as C
+
+    from morphoshell_amd import native
+
+    name = C.create_string_buffer(64)
+    cus, hbm, clk, bus = C.c_int32(0), C.c_int64(0), C.c_int32(0), C.c_int32(0)
+    native.check(native.lib().ms_device_info(name, 64, C.byref(cus), C.byref(hbm), C.byref(clk), C.byref(bus)))
+    assert name.value and len(name.value) < 64
+    assert cus.value > 0 and hbm.value > 0 and clk.value > 0 and bus.value > 0
+    # a short buffer is truncated and terminated; null outputs are allowed
+    short = C.create_string_buffer(b"x" * 4, 4)
+    native.check(native.lib().ms_device_info(short, 4, None, None, None, None))
+    assert short.raw[3:4] == b"\0" and short.value == name.value[:3]
