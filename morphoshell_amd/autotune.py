@@ -5,7 +5,7 @@ steps per captured HIP graph and the step form (triangle + node launches, or
 the fused owner-computes tile launch) for a context, timing each candidate with the
 library's own HIP events, and caches the winner by workload signature (total
 triangles and nodes bucketed to powers of two, shell count bucket, precision,
-device) in a JSON file (``$MS_TUNE_CACHE`` or
+device name and compute units) in a JSON file (``$MS_TUNE_CACHE`` or
 ``~/.cache/morphoshell_amd/tuning.json``).  Results do not depend on the
 launch shape (the gather order is fixed by the CSR), so tuning never changes
 numbers, only time.
@@ -53,13 +53,33 @@ def _cache_path() -> str:
     return os.path.join(os.path.expanduser("~"), ".cache", "morphoshell_amd", "tuning.json")
 
 
+_DEVICE = None
+
+
+def _device() -> str:
+    """The device's name as the runtime reports it (its architecture name on
+    parts without a marketing name), with its compute-unit count."""
+    global _DEVICE
+    if _DEVICE is None:
+        import ctypes as C
+
+        from . import native
+
+        name = C.create_string_buffer(128)
+        cus = C.c_int32(0)
+        native.check(native.lib().ms_device_info(name, 128, C.byref(cus), None, None, None))
+        dev = name.value.decode(errors="replace").split(":")[0].replace(" ", "_")
+        _DEVICE = f"{dev}x{cus.value}"
+    return _DEVICE
+
+
 def signature(ctx: Context) -> str:
     T = sum(s.T for s in ctx.shells)
     N = sum(s.N for s in ctx.shells)
     S = len(ctx.shells)
     b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
     prec = "mixed" if ctx.precision == 1 else "fp64"
-    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-v3"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-{_device()}-v4"
 
 
 def _load() -> dict:

@@ -502,6 +502,7 @@ def test_autotuner_picks_a_shape_without_changing_results(tmp_path, monkeypatch)
     with _ctx(shells) as a, _ctx(shells) as b:
         best = autotune.tune(a, steps=10, space=space)
         assert not best["cached"] and best["candidates"] == 8
+        assert autotune._device() in best["signature"]  # cached per device
         t = a.get_tuning()
         assert t["fused"] == best["fused"] and t["node_block"] == best["node_block"]
         again = autotune.tune(b, steps=10, space=space)
