@@ -265,12 +265,14 @@ def bench_single(args) -> dict:
     states = [(x, v)]
     # the product form is the fused tile step (256-node tiles); the
     # triangle + node launch form is timed beside it for reference
-    forms = {"fused": dict(node_block=256, fused=1), "two_launch": dict(tri_block=256, node_block=256, fused=0)}
+    # the fused form at its fastest shape for this shell: 512-node tiles, two workgroups per CU
+    forms = {"fused": dict(node_block=512, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2),
+             "two_launch": dict(tri_block=256, node_block=256, fused=0, tri_grid_per_cu=0, node_grid_per_cu=0)}
     res = {}
     for prec in args.precisions.split(","):
         with Context(sh, precision=prec) as ctx:
             for form in args.forms.split(","):
-                ctx.set_tuning(tri_grid_per_cu=0, node_grid_per_cu=0, **forms[form])
+                ctx.set_tuning(**forms[form])
                 secs = time_steps(ctx, states, args.steps, args.warmup)
                 kt = kernel_times(ctx, states, min(args.steps, 50))
                 res[(prec, form)] = {"secs": secs, "kt": kt}
@@ -379,8 +381,8 @@ def bench_single(args) -> dict:
 def batch_traffic(tuned: dict, kernel_ms: float):
     """The fused batch kernel's HBM bytes per launch from the committed
     counter summary (same batch and tuning) over this run's kernel time."""
-    if not tuned.get("fused") or tuned.get("node_block") != 256:
-        return None
+    if not tuned.get("fused") or tuned.get("node_block") != 512 or tuned.get("tile_chunks", 1) != 1:
+        return None  # the committed counters are of 512-node tiles
     t = measured_traffic("tile_step_kernel<true, false, false, true>", BATCH_TRAFFIC_PROFILE)
     if not t:
         return None
@@ -405,10 +407,10 @@ def bench_batch(args) -> dict:
         for i, (x, v) in enumerate(states):
             ctx.set_state(i, x, v)
         if args.no_autotune:
-            # the shape the autotuner picks for this batch
-            tuned = {"tri_block": 256, "node_block": 256, "tile_chunks": 1, "grid_per_cu": 4, "graph_steps": 16,
+            # the fastest shape measured for this batch
+            tuned = {"tri_block": 256, "node_block": 512, "tile_chunks": 1, "grid_per_cu": 0, "graph_steps": 16,
                      "fused": 1, "cached": None}
-            ctx.set_tuning(tri_block=256, node_block=256, tri_grid_per_cu=4, node_grid_per_cu=4, graph_steps=16,
+            ctx.set_tuning(tri_block=256, node_block=512, tri_grid_per_cu=0, node_grid_per_cu=0, graph_steps=16,
                            fused=1, tile_chunks=1)
         else:
             tuned = autotune.tune(ctx, steps=20, use_cache=False, verbose=args.verbose)

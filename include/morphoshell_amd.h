@@ -322,7 +322,7 @@ int ms_resume_shell(ms_ctx* ctx, int32_t shell);
  * size or graph_steps of 0, or a grid_per_cu of -1, keeps the current value. */
 typedef struct ms_tuning {
     int32_t tri_block;        /* threads (= triangles) per block: 64, 128 or 256 */
-    int32_t node_block;       /* threads (= nodes) per block: 64, 128 or 256 */
+    int32_t node_block;       /* threads (= nodes) per block: 64, 128, 256 or 512 */
     int32_t tri_grid_per_cu;  /* persistent grid: blocks per CU; 0 = one block per chunk */
     int32_t node_grid_per_cu;
     int32_t graph_steps;      /* steps captured per HIP graph */

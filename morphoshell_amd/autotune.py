@@ -19,6 +19,7 @@ import math
 import os
 import time
 
+from .native import NativeError
 from .shell import Context
 
 DEFAULT_SPACE = {
@@ -26,7 +27,7 @@ DEFAULT_SPACE = {
     "node_block": (128, 256),
     "grid_per_cu": (0, 4, 8),
     "graph_steps": (16,),
-    "fused_node_block": (128, 256),  # block sizes of the fused step
+    "fused_node_block": (128, 256, 512),  # block sizes of the fused step
     "fused_tile_chunks": (1, 2),      # node chunks per tile of the fused step
 }
 
@@ -128,7 +129,12 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
     sp = space or DEFAULT_SPACE
     results = []
     for cand in _candidates(sp):
-        _apply(ctx, cand)
+        try:
+            _apply(ctx, cand)
+        except NativeError as e:  # a shape the library refuses for this context (e.g. tile LDS)
+            if verbose:
+                print(f"[autotune] {sig} {cand}: refused ({e})", file=sys.stderr)
+            continue
         t = time_config(ctx, steps)
         results.append((t, cand))
         if verbose:

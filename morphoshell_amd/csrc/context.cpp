@@ -1393,7 +1393,8 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
         n.tri_block = t->tri_block;
     }
     if (t->node_block) {
-        if (!okblock(t->node_block)) return fail(c, MS_ERR_ARG, "node_block must be 64, 128 or 256");
+        if (!okblock(t->node_block) && t->node_block != 512)
+            return fail(c, MS_ERR_ARG, "node_block must be 64, 128, 256 or 512");
         n.node_block = t->node_block;
     }
     if (t->tri_grid_per_cu >= 0) n.tri_grid_per_cu = t->tri_grid_per_cu;
@@ -1411,7 +1412,6 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
     const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
     const bool retile = rechunk || n.tile_chunks != c->tune.tile_chunks;
     if (int r = pull_shells(c)) return r;
-    const ms_tuning old = c->tune;
     c->tune = n;
     destroy_graph(c);
     if (rechunk) {
@@ -1421,7 +1421,9 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
     if (retile) c->tiles_ready = false;
     if (n.fused == 1 && !c->tiles_ready) {
         if (int r = build_tiles(c)) {
-            c->tune.fused = old.fused;  // keep a working configuration
+            // the tiles are gone: the two-launch form, which needs none, keeps
+            // the context working (the chunks have the new block size)
+            c->tune.fused = 0;
             return r;
         }
     }

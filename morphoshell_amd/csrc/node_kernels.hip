@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
 
 // ----------------------------------------------------------- fused step
 template <bool MIXED, bool CHECK>
-__global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_step_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void node_step_kernel(NodeKArgs k) {
 
 // ------------------------------------------------------------ ops kernels
 template <bool MIXED>
-__global__ __launch_bounds__(256) void node_gather_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_gather_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void node_gather_kernel(NodeKArgs k) {
     }
 }
 
-__global__ __launch_bounds__(256) void node_damping_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_damping_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void node_damping_kernel(NodeKArgs k) {
     }
 }
 
-__global__ __launch_bounds__(256) void node_clamp_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_clamp_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
@@ -367,7 +367,7 @@ __global__ void fill_ll_kernel(long long* p, int n, long long v) {
 }
 
 // First offending dof (n_local + c * N_shell) per shell, deterministic min.
-__global__ __launch_bounds__(256) void node_scan_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_scan_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void node_scan_kernel(NodeKArgs k) {
     }
 }
 
-__global__ __launch_bounds__(256) void node_advance_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_advance_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void node_advance_kernel(NodeKArgs k) {
     }
 }
 
-__global__ __launch_bounds__(256) void node_equil_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_equil_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void node_equil_kernel(NodeKArgs k) {
     }
 }
 
-__global__ __launch_bounds__(256) void node_kinetic_kernel(NodeKArgs k) {
+__global__ __launch_bounds__(512) void node_kinetic_kernel(NodeKArgs k) {
     const int N = k.nd.N;
     for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
         const int ci = xcd_remap(vi, k.nchunks);

@@ -86,7 +86,7 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
 // every wave's record in flight without spilling, and measured faster than at
 // 3 waves (168 registers, where the loads had to wait for registers).
 template <bool MIXED, bool CHECK, bool LCE, bool UNI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 2) : (MIXED ? 4 : 2))))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(LCE ? (MIXED ? 3 : 2) : (MIXED ? 4 : 2))))
 void tile_step_kernel(TileKArgs k) {
     typedef typename std::conditional<MIXED, float, double>::type CT;
     extern __shared__ double smem[];

@@ -317,7 +317,7 @@ def test_fused_equilibrium_check_and_halt():
 # ------------------------------------------------- fused owner-computes step
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("node_block,grid,tile_chunks", [(64, 0, 1), (128, 2, 1), (256, 0, 1), (64, 0, 2),
-                                                         (256, 3, 2)])
+                                                         (256, 3, 2), (512, 0, 1), (512, 2, 1)])
 def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid, tile_chunks):
     """The fused tile launch sums the same contributions in the same
     patch-incidence order as the two-launch path: states agree bit for bit,
@@ -780,3 +780,23 @@ def test_device_info_reports_the_device():
     short = C.create_string_buffer(b"x" * 4, 4)
     native.check(native.lib().ms_device_info(short, 4, None, None, None, None))
     assert short.raw[3:4] == b"\0" and short.value == name.value[:3]
+
+
+def test_refused_tile_shape_leaves_a_working_two_launch_context(sh):
+    """A fused shape whose tiles do not fit the LDS (1024-node FP64 tiles
+    here) is refused with an error; the context falls back to the two-launch
+    form at the new block size and steps exactly as a context set to it."""
+    from morphoshell_amd.native import NativeError
+
+    x, v = deformed_state(sh, seed=41)
+    with _ctx(sh) as a, _ctx(sh) as b:
+        a.set_tuning(node_block=256, fused=1)
+        with pytest.raises(NativeError, match="LDS"):
+            a.set_tuning(node_block=512, fused=1, tile_chunks=2)
+        t = a.get_tuning()
+        assert t["fused"] == 0 and t["node_block"] == 512
+        b.set_tuning(node_block=512, fused=0)
+        for c in (a, b):
+            c.set_state(0, x, v)
+            c.step(20)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
