@@ -238,7 +238,7 @@ def baseline_parity(sh, x, v, n: int = 20) -> dict:
     Eo = sysm.energies(x, v)
     parity["fp64_energy_rel_err"] = float(np.abs(E[:2] - Eo[:2]).max() / max(abs(Eo[:2]).max(), 1e-300))
     with Context(sh, precision="fp64") as ctx:
-        ctx.set_tuning(node_block=256, fused=1)
+        ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
         ctx.set_state(0, x, v)
         ctx.step(n)
         xg = ctx.get_state(0)[0]
@@ -639,7 +639,7 @@ def bench_relax_large(args) -> dict:
     series, energy = {}, {}
     for prec in ("fp64", "mixed"):
         with Context(sh, precision=prec) as ctx:
-            ctx.set_tuning(node_block=256, fused=1)
+            ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
             ctx.set_state(0, x0, v0)
             ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0,
                             equil_interval=interval, tol_f=1e-6, tol_v=1e-5)
@@ -706,7 +706,7 @@ def _relax_series(sh, precision, x, v, K, steps, chunk, from_ansatz=False):
         if from_ansatz:
             a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
             ctx.set_zero_dial_geometry(0, a.reshape(-1), b.reshape(-1))
-        ctx.set_tuning(node_block=256, fused=1)
+        ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
         ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=1000,
                         tol_f=1e-6, tol_v=1e-5)
         ctx.synchronize()

@@ -396,11 +396,28 @@ int run(int argc, char** argv) {
         r.log << "Error: " << ms_last_error() << std::endl;
         return -1;
     }
+    // Launch shape by mesh size, from measured step times on discs of 18k to
+    // 7.3M triangles: small meshes are launch-bound and fastest as a triangle
+    // and a node launch of 64-wide blocks; from ~100k triangles the fused step
+    // on 256-node tiles; from ~1M triangles on 512-node tiles (enough tiles to
+    // fill the device).  Every shape gives the same bits.
     ms_tuning tune{};
     tune.tri_grid_per_cu = -1;
     tune.node_grid_per_cu = -1;
-    tune.fused = 1;
-    if (ms_set_tuning(r.ctx, &tune) != MS_OK) r.log << "Fused step unavailable: " << ms_ctx_last_error(r.ctx) << std::endl;
+    if (T < 100000) {
+        tune.tri_block = tune.node_block = 64;
+        tune.tri_grid_per_cu = tune.node_grid_per_cu = 8;
+        tune.fused = 0;
+    } else if (T < 1000000) {
+        tune.node_block = 256;
+        tune.fused = 1;
+    } else {
+        tune.node_block = 512;
+        tune.node_grid_per_cu = 2;
+        tune.fused = 1;
+    }
+    if (ms_set_tuning(r.ctx, &tune) != MS_OK)
+        r.log << "Launch shape unavailable: " << ms_ctx_last_error(r.ctx) << std::endl;
     check(ms_set_state(r.ctx, 0, dofs.data(), nullptr), "ms_set_state");
     if (r.st.dialling_from_ansatz_rather_than_ref_state) {
         // main.cpp:321-329: a and b of the start state become the zero-dial geometry
