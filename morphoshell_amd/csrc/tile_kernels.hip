@@ -196,9 +196,9 @@ void tile_step_kernel(TileKArgs k) {
                     store(j, reinterpret_cast<const CT*>(out));
                 }
             } else {
-                // one entry at a time: a second record in flight (entries j and
-                // j + B together) needs more than the 256 registers of 2 waves
-                // per SIMD and measured 25% slower
+                // one entry at a time: the next entry's record loaded before this
+                // one's mechanics needs all 256 registers of 2 waves per SIMD and
+                // spills (40 B per lane), and measured 24% slower
                 for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
                     unsigned lp[6];
                     patch(j, lp);
