@@ -317,9 +317,15 @@ int ms_relax_run(ms_ctx* ctx, int64_t max_steps, int64_t poll_every, ms_relax_re
 /* Un-halt a shell (e.g. after the driver handled an equilibrium event). */
 int ms_resume_shell(ms_ctx* ctx, int32_t shell);
 
-/* Launch-shape knobs of the per-step kernels.  The autotuner in the Python
+/* ------------------------------------------------ device-path controls
+ * The declarations from here to the warm start have no counterpart in the
+ * reference: launch shapes, timing and measurement aids of the device path.
+ *
+ * Launch-shape knobs of the per-step kernels.  The autotuner in the Python
  * package searches these and caches the choice.  In ms_set_tuning a block
- * size or graph_steps of 0, or a grid_per_cu of -1, keeps the current value. */
+ * size or graph_steps of 0, or a grid_per_cu of -1, keeps the current value.
+ * A fused shape whose tile does not fit the LDS is refused (MS_ERR_ARG); the
+ * context then keeps the new block sizes in the two-launch form (fused 0). */
 typedef struct ms_tuning {
     int32_t tri_block;        /* threads (= triangles) per block: 64, 128 or 256 */
     int32_t node_block;       /* threads (= nodes) per block: 64, 128, 256 or 512 */
@@ -334,7 +340,7 @@ typedef struct ms_tuning {
     int32_t tile_chunks;      /* node chunks per tile of the fused step: 1 or 2
                                  (larger tiles recompute fewer halo triangles);
                                  0 keeps */
-    int32_t reserved;
+    int32_t reserved;         /* 0 */
 } ms_tuning;
 int ms_set_tuning(ms_ctx* ctx, const ms_tuning* t);
 int ms_get_tuning(const ms_ctx* ctx, ms_tuning* t);
