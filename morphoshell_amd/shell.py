@@ -282,6 +282,8 @@ class Context:
             shells = [shells]
         self.shells = list(shells)
         prec = precision or self.shells[0].settings.precision
+        if prec not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {prec!r}")
         self.precision = nat.MS_MIXED if prec == "mixed" else nat.MS_FP64
         L = nat.lib()
         descs = (nat.ShellDesc * len(self.shells))()
@@ -346,25 +348,38 @@ class Context:
     def _ck(self, code: int) -> None:
         nat.check(code, self.h)
 
+    def _vec(self, a, n: int, what: str) -> np.ndarray:
+        """a as a contiguous float64 array of n values (the C-ABI reads n)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != n:
+            raise ValueError(f"{what}: expected {n} values, got {a.size}")
+        return a
+
+    def _shell(self, shell: int):
+        if not 0 <= shell < len(self.shells):
+            raise IndexError(f"shell {shell} out of range (0..{len(self.shells) - 1})")
+        return self.shells[shell]
+
     # -- state
     def set_state(self, shell: int, dofs: np.ndarray, vel: np.ndarray | None = None) -> None:
-        dofs = np.ascontiguousarray(dofs, dtype=np.float64)
-        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        n = 3 * self._shell(shell).N
+        dofs = self._vec(dofs, n, "set_state dofs")
+        vel = None if vel is None else self._vec(vel, n, "set_state velocities")
         self._ck(self.L.ms_set_state(self.h, shell, nat.dptr(dofs), nat.dptr(vel)))
 
     def get_state(self, shell: int = 0) -> tuple[np.ndarray, np.ndarray]:
-        n = self.shells[shell].N
+        n = self._shell(shell).N
         x, v = np.empty(3 * n), np.empty(3 * n)
         self._ck(self.L.ms_get_state(self.h, shell, nat.dptr(x), nat.dptr(v)))
         return x, v
 
     def get_forces(self, shell: int = 0) -> np.ndarray:
-        f = np.empty(3 * self.shells[shell].N)
+        f = np.empty(3 * self._shell(shell).N)
         self._ck(self.L.ms_get_forces(self.h, shell, nat.dptr(f)))
         return f
 
     def set_forces(self, shell: int, f: np.ndarray) -> None:
-        f = np.ascontiguousarray(f, dtype=np.float64)
+        f = self._vec(f, 3 * self._shell(shell).N, "set_forces")
         self._ck(self.L.ms_set_forces(self.h, shell, nat.dptr(f)))
 
     def set_dial_clock(self, shell: int, last_eq: float, spacing: float, steps_per_phase: int, phase_steps: int,
@@ -382,11 +397,14 @@ class Context:
         self._ck(self.L.ms_set_shell_params(self.h, shell, C.byref(p)))
 
     def set_zero_dial_geometry(self, shell: int, a0: np.ndarray | None, b0: np.ndarray | None) -> None:
-        a0 = None if a0 is None else np.ascontiguousarray(a0, dtype=np.float64)
-        b0 = None if b0 is None else np.ascontiguousarray(b0, dtype=np.float64)
+        n = 3 * self._shell(shell).T
+        a0 = None if a0 is None else self._vec(a0, n, "set_zero_dial_geometry a0")
+        b0 = None if b0 is None else self._vec(b0, n, "set_zero_dial_geometry b0")
         self._ck(self.L.ms_set_zero_dial_geometry(self.h, shell, nat.dptr(a0), nat.dptr(b0)))
 
     def set_precision(self, precision: str) -> None:
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.precision = nat.MS_MIXED if precision == "mixed" else nat.MS_FP64
         self._ck(self.L.ms_set_precision(self.h, self.precision))
 
@@ -418,7 +436,7 @@ class Context:
 
     def is_at_equilibrium(self, shell: int = 0, force_tol: float | None = None,
                           speed_tol: float | None = None) -> dict:
-        st = self.shells[shell].settings
+        st = self._shell(shell).settings
         ft = st.force_to_char_force_ratio_equil_threshold if force_tol is None else force_tol
         vt = st.speed_to_char_speed_ratio_equil_threshold if speed_tol is None else speed_tol
         r = nat.StepReport()
@@ -431,13 +449,13 @@ class Context:
         return out
 
     def calc_a_comps_and_b_comps_and_normals(self, shell: int = 0):
-        T = self.shells[shell].T
+        T = self._shell(shell).T
         a, b, n = np.empty(3 * T), np.empty(3 * T), np.empty(3 * T)
         self._ck(self.L.ms_calc_geometry(self.h, shell, nat.dptr(a), nat.dptr(b), nat.dptr(n)))
         return a.reshape(3, T), b.reshape(3, T), n.reshape(3, T)
 
     def calc_energy_densities(self, shell: int = 0) -> np.ndarray:
-        T = self.shells[shell].T
+        T = self._shell(shell).T
         d = np.empty(2 * T)
         self._ck(self.L.ms_calc_energy_densities(self.h, shell, nat.dptr(d)))
         return d.reshape(2, T)
@@ -445,7 +463,7 @@ class Context:
     def calc_diagnostics(self, shell: int = 0) -> dict:
         """Gauss and mean curvature and the strain measure per triangle
         (calc_curvatures.cpp:35-94, calc_strains.cpp:35-78)."""
-        T = self.shells[shell].T
+        T = self._shell(shell).T
         g, m, s = np.empty(T), np.empty(T), np.empty(T)
         self._ck(self.L.ms_calc_diagnostics(self.h, shell, nat.dptr(g), nat.dptr(m), nat.dptr(s)))
         return {"gauss_curv": g, "mean_curv": m, "strain": s}
@@ -471,7 +489,7 @@ class Context:
     def relax_setup(self, shell: int, steps_per_dial_phase: int | None = None, dial_spacing: float | None = None,
                     dial_start: float = 0.0, equil_interval: int | None = None, tol_f: float | None = None,
                     tol_v: float | None = None, pause_on_eq: bool = False) -> None:
-        sh = self.shells[shell]
+        sh = self._shell(shell)
         st, sc = sh.settings, sh.scales
         p = nat.RelaxParams()
         p.tol_f = st.force_to_char_force_ratio_equil_threshold if tol_f is None else tol_f

@@ -800,3 +800,27 @@ def test_refused_tile_shape_leaves_a_working_two_launch_context(sh):
             c.set_state(0, x, v)
             c.step(20)
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+
+
+def test_wrapper_checks_array_sizes_and_shell_indices(sh):
+    """The C-ABI reads as many values as the shell has; the Python wrapper
+    rejects arrays of another size and shell indices out of range before
+    any call."""
+    x, v = deformed_state(sh, seed=42)
+    with _ctx(sh) as ctx:
+        with pytest.raises(ValueError, match="expected"):
+            ctx.set_state(0, x[:-1], v)
+        with pytest.raises(ValueError, match="expected"):
+            ctx.set_state(0, x, v[:-3])
+        with pytest.raises(ValueError, match="expected"):
+            ctx.set_forces(0, np.zeros(3 * sh.N + 3))
+        with pytest.raises(ValueError, match="expected"):
+            ctx.set_zero_dial_geometry(0, np.ones(3 * sh.T - 1), None)
+        with pytest.raises(IndexError):
+            ctx.get_state(1)
+        with pytest.raises(IndexError):
+            ctx.set_state(-1, x, v)
+        with pytest.raises(ValueError, match="precision"):
+            ctx.set_precision("fp32")
+        ctx.set_state(0, x, v)  # the right sizes still work
+        assert np.array_equal(ctx.get_state(0)[0], x)

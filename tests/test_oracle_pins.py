@@ -428,3 +428,14 @@ def test_c_abi_library_exports_header_symbols():
     h = ctypes.c_void_p()
     assert L.ms_create(None, 0, 0, ctypes.byref(h)) == native.MS_ERR_ARG
     assert b"bad arguments" in L.ms_last_error()
+
+
+def test_context_rejects_an_unknown_precision_before_any_device_call():
+    import pytest
+
+    from morphoshell_amd.meshing import disc_mesh
+    from morphoshell_amd.shell import Context, Settings, prepare_shell
+
+    sh = prepare_shell(disc_mesh(0.2), Settings())
+    with pytest.raises(ValueError, match="precision"):
+        Context(sh, precision="fp32")
