@@ -473,8 +473,12 @@ def test_create_and_call_argument_errors(sh):
     with pytest.raises(NativeError, match="3-column"):
         _ctx(bad)
     with _ctx(sh) as ctx:
-        with pytest.raises(NativeError, match="out of range"):
-            ctx.set_state(1, sh.ref_pos)
+        # the library's own check, below the wrapper's (which raises IndexError)
+        from morphoshell_amd import native as nat
+
+        x0 = np.ascontiguousarray(sh.ref_pos, dtype=np.float64)
+        assert ctx.L.ms_set_state(ctx.h, 1, nat.dptr(x0), None) == MS_ERR_ARG
+        assert b"out of range" in ctx.L.ms_ctx_last_error(ctx.h)
         with pytest.raises(NativeError, match="fused must be"):
             ctx.set_tuning(fused=3)
         with pytest.raises(NativeError, match="node_block"):
