@@ -828,3 +828,32 @@ def test_wrapper_checks_array_sizes_and_shell_indices(sh):
             ctx.set_precision("fp32")
         ctx.set_state(0, x, v)  # the right sizes still work
         assert np.array_equal(ctx.get_state(0)[0], x)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_halted_shells_fused_active_tiles(precision):
+    """With the fused form, after some shells halt the running shells' tiles
+    run from the active-tile list, with more tiles than CUs and with fewer.
+    The running shells' states equal, bit for bit, those of a context where
+    nothing halts."""
+    shells = [_shell(0.04, seed=i, h=0.02, kappa=0.3 - 0.05 * i) for i in range(24)]
+    states = [deformed_state(s, seed=300 + i) for i, s in enumerate(shells)]
+    with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        for c in (a, b):
+            c.set_tuning(node_block=64, tri_block=64, fused=1)
+            for i, st in enumerate(states):
+                c.set_state(i, *st)
+        running = set(range(len(shells)))
+        for halting in (range(0, 4), range(4, 20)):  # ~720 tiles left running, then ~144
+            for i in halting:
+                a.relax_setup(i, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=2,
+                              tol_f=1e30, tol_v=1e30)
+            ra = a.step(6)
+            b.step(6)
+            assert all(ra[i]["halted"] for i in halting)
+            running -= set(halting)
+            a.step(20)
+            b.step(20)
+            for i in running:
+                assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
+                assert np.array_equal(a.get_state(i)[1], b.get_state(i)[1]), i
