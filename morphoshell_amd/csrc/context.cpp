@@ -115,11 +115,15 @@ hipError_t dalloc(ms_ctx* c, T** p, size_t n) {
     return e;
 }
 
-int build_chunks(ms_ctx* c) {
-    const int tb = c->tune.tri_block, nb = c->tune.node_block;
+// Chunk tables for blocks of tb triangles and nb nodes, all-or-nothing: the
+// new tables and partial buffers are allocated and filled before the old
+// ones are freed, so a failed allocation leaves the context on its current
+// tables (and its current tiles and graph, which name them).
+int build_chunks(ms_ctx* c, int tb, int nb) {
     std::vector<Chunk> tc, nc;
+    std::vector<ShellDev> hs = c->hs;
     for (int s = 0; s < c->num_shells; ++s) {
-        ShellDev& S = c->hs[s];
+        ShellDev& S = hs[s];
         S.tri_chunk_begin = (int)tc.size();
         for (int t = S.tri_begin; t < S.tri_begin + S.num_tris; t += tb)
             tc.push_back(make_int4(t, std::min(t + tb, S.tri_begin + S.num_tris), s, (int)tc.size()));
@@ -129,31 +133,41 @@ int build_chunks(ms_ctx* c) {
             nc.push_back(make_int4(n, std::min(n + nb, S.node_begin + S.num_nodes), s, (int)nc.size()));
         S.node_chunk_count = (int)nc.size() - S.node_chunk_begin;
     }
-    if (c->d_tri_chunks) hipFree(c->d_tri_chunks);
-    if (c->d_node_chunks) hipFree(c->d_node_chunks);
-    if (c->b.part) hipFree(c->b.part);
-    if (c->b.part_i) hipFree(c->b.part_i);
-    if (c->d_act_tri) hipFree(c->d_act_tri);
-    if (c->d_act_node) hipFree(c->d_act_node);
-    c->d_tri_chunks = c->d_node_chunks = c->d_act_tri = c->d_act_node = nullptr;
-    c->b.part = nullptr;
-    c->b.part_i = nullptr;
-    c->act_live.clear();
-    c->act_all = true;
-    HIPC(c, hipMalloc((void**)&c->d_tri_chunks, tc.size() * sizeof(Chunk)));
-    HIPC(c, hipMalloc((void**)&c->d_node_chunks, nc.size() * sizeof(Chunk)));
-    HIPC(c, hipMalloc((void**)&c->d_act_tri, tc.size() * sizeof(Chunk)));
-    HIPC(c, hipMalloc((void**)&c->d_act_node, nc.size() * sizeof(Chunk)));
-    HIPC(c, hipMemcpy(c->d_tri_chunks, tc.data(), tc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
-    HIPC(c, hipMemcpy(c->d_node_chunks, nc.data(), nc.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    const size_t part_len = 4 * std::max(tc.size(), nc.size());
+    Chunk *dt = nullptr, *dn = nullptr, *at = nullptr, *an = nullptr;
+    double* part = nullptr;
+    int* part_i = nullptr;
+    hipError_t e = hipMalloc((void**)&dt, tc.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&dn, nc.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&at, tc.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&an, nc.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&part, part_len * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&part_i, part_len * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(dt, tc.data(), tc.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, nc.data(), nc.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        for (void* p : {(void*)dt, (void*)dn, (void*)at, (void*)an, (void*)part, (void*)part_i})
+            if (p) hipFree(p);
+        return fail(c, MS_ERR_HIP, fmt("chunk tables: %s", hipGetErrorString(e)));
+    }
+    for (void* p : {(void*)c->d_tri_chunks, (void*)c->d_node_chunks, (void*)c->d_act_tri, (void*)c->d_act_node,
+                    (void*)c->b.part, (void*)c->b.part_i})
+        if (p) hipFree(p);
+    c->d_tri_chunks = dt;
+    c->d_node_chunks = dn;
+    c->d_act_tri = at;
+    c->d_act_node = an;
+    c->b.part = part;
+    c->b.part_i = part_i;
+    c->part_len = part_len;
     c->n_tri_chunks = (int)tc.size();
     c->n_node_chunks = (int)nc.size();
-    c->part_len = 4 * (size_t)std::max(tc.size(), nc.size());
     c->h_tri_chunks = std::move(tc);
     c->h_node_chunks = std::move(nc);
-    HIPC(c, hipMalloc((void**)&c->b.part, c->part_len * sizeof(double)));
-    HIPC(c, hipMalloc((void**)&c->b.part_i, c->part_len * sizeof(int)));
-    c->tiles_ready = false;
+    c->hs = std::move(hs);
+    c->act_live.clear();
+    c->act_all = true;
+    c->tiles_ready = false;  // tiles are built over the node chunks
     return MS_OK;
 }
 
@@ -183,20 +197,21 @@ void free_tiles(ms_ctx* c) {
 // the blocked record layout (see RecSlot; gathered on the device; the patch
 // ids are replaced by tile-local ids).
 int build_tile_records(ms_ctx* c) {
-    free_tile_records(c);
     const int T = c->T, Tp = c->tile_T;
     const size_t vals = (size_t)((Tp + 63) / 64) * REC_SLOTS * 64;
-    auto alloc = [&](size_t bytes, void** out) -> hipError_t {
-        hipError_t e = hipMalloc(out, bytes);
-        if (e != hipSuccess) return e;
-        c->tile_allocs.push_back(*out);
-        c->tile_rec_bytes += (int64_t)bytes;
-        c->dev_bytes += (int64_t)bytes;
-        return hipSuccess;
-    };
     void *r64 = nullptr, *r32 = nullptr;
-    HIPC(c, alloc(vals * sizeof(double), &r64));
-    HIPC(c, alloc(vals * sizeof(float), &r32));
+    hipError_t ea = hipMalloc(&r64, vals * sizeof(double));
+    if (ea == hipSuccess) ea = hipMalloc(&r32, vals * sizeof(float));
+    if (ea != hipSuccess) {  // the current records (if any) stay in place
+        if (r64) hipFree(r64);
+        return fail(c, MS_ERR_HIP, fmt("tile records: %s", hipGetErrorString(ea)));
+    }
+    free_tile_records(c);
+    c->tile_allocs = {r64, r32};
+    c->tile_rec_bytes = (int64_t)(vals * (sizeof(double) + sizeof(float)));
+    c->dev_bytes += c->tile_rec_bytes;
+    c->tile.rec64 = (const double*)r64;
+    c->tile.rec32 = (const float*)r32;
     auto gather = [&](const void* in, void* out, int elem_bytes, int ncomp, int slot) {
         if (in) launch_tile_gather(c->stream, in, out, elem_bytes, ncomp, T, Tp, slot, c->tile.tris);
     };
@@ -214,8 +229,6 @@ int build_tile_records(ms_ctx* c) {
     gather(c->hmuA32, r32, 4, 3, RS_HMUA);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
-    c->tile.rec64 = (const double*)r64;
-    c->tile.rec32 = (const float*)r32;
     return MS_OK;
 }
 
@@ -811,7 +824,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     }
     hipMemset(c->b.F, 0, 3 * sizeof(double) * (size_t)N);
     hipMemset(c->d_clock, 0, sizeof(ClockDev));
-    if (build_chunks(c) != MS_OK) {
+    if (build_chunks(c, c->tune.tri_block, c->tune.node_block) != MS_OK) {
         set_last_error(c->err);
         return bail(MS_ERR_HIP);
     }
@@ -949,6 +962,7 @@ extern "C" int ms_set_shell_params(ms_ctx* c, int32_t s, const ms_shell_params* 
 extern "C" int ms_set_zero_dial_geometry(ms_ctx* c, int32_t s, const double* a0, const double* b0) {
     if (int r = shell_ok(c, s)) return r;
     if (int r = pull_shells(c)) return r;
+    destroy_graph(c);  // the kernel arguments change (a0, b0, tile records)
     ShellDev& S = c->hs[s];
     const int T = c->T;
     if (a0) {
@@ -977,9 +991,15 @@ extern "C" int ms_set_zero_dial_geometry(ms_ctx* c, int32_t s, const double* a0,
                               sizeof(double) * S.num_tris, hipMemcpyHostToDevice));
         S.has_b0 = 1;
     }
-    if (c->tiles_ready)  // the tile-ordered copies of a0, b0
-        if (int r = build_tile_records(c)) return r;
-    destroy_graph(c);  // kernel arguments changed
+    if (c->tiles_ready) {  // the tile-ordered copies of a0, b0
+        if (int r = build_tile_records(c)) {
+            // without current records the fused step cannot run: drop the
+            // tiles, and the two-launch form (which needs none) carries on
+            free_tiles(c);
+            c->tune.fused = 0;
+            return r;
+        }
+    }
     return push_shells(c);
 }
 
@@ -1179,6 +1199,8 @@ extern "C" int ms_step(ms_ctx* c, int64_t nsteps, int64_t equil_every, double to
         c->hs[s].checked = 0;
     }
     if (int r = push_shells(c)) return r;
+    if (c->tune.fused == 1 && !c->tiles_ready)
+        return fail(c, MS_ERR_ARG, "ms_step: the fused step has no tiles (ms_set_tuning builds them)");
     if (int r = refresh_active(c)) return r;
     const bool use_graph = !c->timing;
     if (use_graph)
@@ -1412,12 +1434,13 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
     const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
     const bool retile = rechunk || n.tile_chunks != c->tune.tile_chunks;
     if (int r = pull_shells(c)) return r;
-    c->tune = n;
     destroy_graph(c);
     if (rechunk) {
-        if (int r = build_chunks(c)) return r;
+        // on failure the context keeps its current tables, tiles and tuning
+        if (int r = build_chunks(c, n.tri_block, n.node_block)) return r;
         if (int r = push_shells(c)) return r;
     }
+    c->tune = n;
     if (retile) c->tiles_ready = false;
     if (n.fused == 1 && !c->tiles_ready) {
         if (int r = build_tiles(c)) {

@@ -169,7 +169,12 @@ struct TileData {
 // slot `slot + k` of every entry j < Tp with tris[j] >= 0 (elem_bytes 4 or 8).
 void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp, int slot,
                         const int* tris);
-int tile_lds_bytes(const TileData& tl, bool mixed);
+// Dynamic LDS of a tile launch: positions of the tile's own and halo nodes
+// (24 B each), in mixed mode the own nodes' running sums (24 B each), and one
+// pass of contributions (3 values per entry).
+inline int tile_lds_bytes(const TileData& tl, bool mixed) {
+    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8);
+}
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.
 hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);
 // One fused step of every shell not halted (the work of launch_tri_forces +

@@ -335,10 +335,6 @@ void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes
                            ncomp, T, Tp, slot, tris);
 }
 
-int tile_lds_bytes(const TileData& tl, bool mixed) {
-    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8);
-}
-
 template <bool MIXED>
 hipError_t raise_lds(int bytes) {
     if (bytes <= 65536) return hipSuccess;
