@@ -4,10 +4,14 @@ programmed metric (abar, bbar, h).
 
 It replaces the ansatz start of the reference
 (move_nodes_to_ansatz_positions.cpp:26-39 with positions read by
-read_vtk_data.cpp:380-445): the predicted positions become the start state
-at dial factor 1, the plain damped dynamics finish the relaxation and judge
-it by residual force, and a flat start with noise is used instead whenever
-the prediction has a higher elastic energy than that flat start.
+read_vtk_data.cpp:380-445): the predicted positions become the start state,
+dialled by default from their own geometry (the reference's
+dialling_from_ansatz_rather_than_ref_state, do_dialling.cpp:194-228: the
+start's a and b are the targets at dial factor 0, so it starts force-free)
+or, with start="direct", started at dial factor 1.  The plain damped
+dynamics finish the relaxation and judge it by residual force, and a flat
+start with noise is used instead whenever the prediction has a higher
+elastic energy than that flat start.
 
 The network itself (forward, backward, Adam) runs on the GPU in the native
 library (``ms_nn_*``, BF16 matrix-core GEMMs with FP32 accumulation).  This

@@ -52,18 +52,20 @@ def _small_batch():
     return shells, graphs, batch_graphs(graphs)
 
 
-def test_forward_and_gradients_match_reference():
+@pytest.mark.parametrize("width,layers", [(32, 2), (64, 4)])
+def test_forward_and_gradients_match_reference(width, layers):
+    """At the configured architecture (4 layers, width 64) and a smaller one."""
     from morphoshell_amd.warmstart import EDGE_FEATURES, NODE_FEATURES, WarmStartNet
 
     shells, graphs, b = _small_batch()
-    net = WarmStartNet(seed=3, width=32, layers=2)
+    net = WarmStartNet(seed=3, width=width, layers=layers)
     p = net.params().astype(np.float64)
     net.set_graph(b)
     out = net.forward()
     rng = np.random.default_rng(0)
     tgt = rng.normal(size=out.shape).astype(np.float32) * 0.3
     loss_g, grad_g = net.loss_backward(tgt, want_grads=True)
-    ref = mpnn_ref.Net(NODE_FEATURES, EDGE_FEATURES, 32, 2, bf16_ops=True)
+    ref = mpnn_ref.Net(NODE_FEATURES, EDGE_FEATURES, width, layers, bf16_ops=True)
     c = {}
     out_r = ref.forward(p, b["node_feat"].astype(np.float64), b["edge_feat"].astype(np.float64), b["src"], b["dst"],
                         c)
@@ -157,3 +159,39 @@ def test_warm_start_dialled_from_its_own_geometry_starts_force_free():
     print(f"force at dial factor 0: {np.abs(F0).max() / scale:.2e} of the programmed forces")
     assert np.abs(F0).max() <= 1e-9 * scale
     assert np.abs(F1 - Fr).max() <= 1e-12 * scale
+
+
+def test_pretrained_network_on_unseen_shells_saves_steps():
+    """The shipped parameters (4 layers, width 64, trained on equilibria of the
+    defect family the batched solver generated) applied to 24 shells of that
+    family from seeds outside the training set: the energy test keeps most
+    predictions, every kept start has a lower elastic energy at dial factor 1
+    than the flat start with noise, and over the shells both starts bring to
+    equilibrium (residual force and speed 1e-5 of their scales) the warm
+    start takes fewer steps than the flat start in the median (steps ratio
+    flat / warm above 1).  The runs are deterministic, so the ratio is a
+    fixed number of this code and these seeds; it is printed."""
+    from morphoshell_amd import warmstart as ws
+
+    net = ws.WarmStartNet.load(ws.PRETRAINED_DEFECT)
+    assert net.arch == (64, 4)
+    shells = ws.training_family(24, 987_000, family="defect")
+    plans, used = ws.warm_start_plans(net, shells)
+    n_used = sum(u is not None for u in used)
+    from morphoshell_amd.workloads import flat_noisy_state
+
+    flats = [flat_noisy_state(sh, seed=i)[0] for i, sh in enumerate(shells)]
+    e_flat = ws.full_dial_energies(shells, flats)
+    e_warm = ws.full_dial_energies(shells, [p[0] if p else f for p, f in zip(plans, flats)])
+    for p, ew, ef in zip(plans, e_warm, e_flat):
+        assert (ew < ef) if p else (ew == ef)
+    base, _, tb = ws.relax_batch(shells, 1e-5, 600_000)
+    warm, _, tw = ws.relax_batch(shells, 1e-5, 600_000, plans=plans)
+    both = [i for i in range(len(shells)) if base[i]["status"] == 2 and warm[i]["status"] == 2]
+    ratio = np.array([(base[i]["eq_step"] + 1) / (warm[i]["eq_step"] + 1) for i in both])
+    print(f"predictions kept {n_used}/24; both converged {len(both)}; steps ratio flat/warm median "
+          f"{np.median(ratio):.3f} (min {ratio.min():.2f}, max {ratio.max():.2f}); {tb:.1f}s flat, {tw:.1f}s warm")
+    net.close()
+    assert n_used >= 18
+    assert len(both) >= 18
+    assert np.median(ratio) > 1.0
