@@ -246,6 +246,17 @@ int ms_calc_energies(ms_ctx* ctx, int32_t shell, double out[3]);
 /* calc_a_comps_and_b_comps_and_normals (calc_a_comps_and_b_comps_and_normals.cpp:34-61):
  * a, b, n as 3T SoA each (any pointer may be NULL). */
 int ms_calc_geometry(ms_ctx* ctx, int32_t shell, double* a, double* b, double* n);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ms_calc_geometry of one shell, computed into a static
+ * device scratch array of the library instead of the output buffer that
+ * ms_calc_geometry allocates on its first call; this call allocates
+ * nothing.  The array holds MS_NOALLOC_MAX_TRIS triangles (36 MiB on each
+ * device the library runs on, held until the process ends); a larger shell
+ * is refused with MS_ERR_ARG.  All calls on one device share the array, so
+ * calls that overlap on one device (from several host threads) race. */
+#define MS_NOALLOC_MAX_TRIS 524288
+int ms_calc_geometry_noalloc(ms_ctx* ctx, int32_t shell, double* a, double* b, double* n);
 /* Per-triangle energy densities (T x 2 SoA: stretch.., bend..), may be NULL. */
 int ms_calc_energy_densities(ms_ctx* ctx, int32_t shell, double* densities);
 /* Output-time diagnostics per triangle (T each, any pointer may be NULL):

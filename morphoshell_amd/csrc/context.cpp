@@ -10,6 +10,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -87,9 +88,43 @@ struct ms_ctx {
 
     int64_t dev_bytes = 0;
     std::vector<void*> allocs;
+
+    // this device's address of the static scratch array of the allocation-free
+    // entry point (null, with the reason in noalloc_err, if it could not be taken)
+    double* noalloc_scratch = nullptr;
+    std::string noalloc_err;
 };
 
 namespace {
+
+// Scratch of ms_calc_geometry_noalloc: a, b and n (3 values each) of up to
+// MS_NOALLOC_MAX_TRIS triangles, 36 MiB in every device's copy of this code
+// object, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: only the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch_for)
+// is passed to kernels and copies.
+constexpr size_t NOALLOC_VALS = 9 * (size_t)MS_NOALLOC_MAX_TRIS;
+static __device__ __align__(256) double scratch[NOALLOC_VALS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev (the current device), taken
+// once per device.
+hipError_t noalloc_scratch_for(int dev, double** out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return hipSuccess;
+}
 
 int fail(ms_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -666,6 +701,10 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // set-up of the allocation-free entry point on this device (the context's
+    // device is current); a failure only disables that entry point
+    if (const hipError_t e = noalloc_scratch_for(dev, &c->noalloc_scratch); e != hipSuccess)
+        c->noalloc_err = fmt("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
     auto bail = [&](int code) {
         ms_destroy(c);
         return code;
@@ -1139,7 +1178,8 @@ extern "C" int ms_calc_geometry(ms_ctx* c, int32_t s, double* a, double* b, doub
     if (int r = ensure_scratch(c)) return r;
     const size_t T = c->T;
     double *ga = c->scratch3, *gb = c->scratch3 + 3 * T, *gn = c->scratch3 + 6 * T;
-    launch_tri_geometry(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, ga, gb, gn);
+    launch_tri_geometry(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells, ga, gb, gn,
+                        0, c->T);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
     const ShellDev& S = c->hs[s];
@@ -1151,6 +1191,33 @@ extern "C" int ms_calc_geometry(ms_ctx* c, int32_t s, double* a, double* b, doub
             HIPC(c, hipMemcpy(outs[q] + (size_t)k * S.num_tris, srcs[q] + k * T + S.tri_begin,
                               sizeof(double) * S.num_tris, hipMemcpyDeviceToHost));
     }
+    return MS_OK;
+}
+
+// ms_calc_geometry of one shell through the static scratch array: no device
+// allocation.  Calls on one device share the array, so calls that overlap on
+// one device (from several host threads) race.
+extern "C" int ms_calc_geometry_noalloc(ms_ctx* c, int32_t s, double* a, double* b, double* n) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!c->noalloc_scratch) return fail(c, MS_ERR_HIP, "ms_calc_geometry_noalloc: " + c->noalloc_err);
+    const ShellDev& S = c->hs[s];  // ranges and chunk tables only (fixed at creation and by re-chunking)
+    if (S.num_tris > MS_NOALLOC_MAX_TRIS)
+        return fail(c, MS_ERR_ARG, fmt("ms_calc_geometry_noalloc: shell %d has %d triangles, above the scratch "
+                                       "limit of %d (MS_NOALLOC_MAX_TRIS); use ms_calc_geometry",
+                                       s, S.num_tris, MS_NOALLOC_MAX_TRIS));
+    const size_t nt = (size_t)S.num_tris;
+    double* ga = c->noalloc_scratch;
+    double* gb = ga + 3 * nt;
+    double* gn = gb + 3 * nt;
+    const Launch L{grid_for(c, S.tri_chunk_count, c->tune.tri_grid_per_cu), c->tune.tri_block, c->stream};
+    launch_tri_geometry(L, tri_data(c), c->b, c->d_tri_chunks + S.tri_chunk_begin, S.tri_chunk_count, c->d_shells,
+                        ga, gb, gn, S.tri_begin, S.num_tris);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double* outs[3] = {a, b, n};
+    const double* srcs[3] = {ga, gb, gn};
+    for (int q = 0; q < 3; ++q)
+        if (outs[q]) HIPC(c, hipMemcpy(outs[q], srcs[q], 3 * nt * sizeof(double), hipMemcpyDeviceToHost));
     return MS_OK;
 }
 

@@ -62,8 +62,10 @@ void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, con
 // partial sums of A*W_s and A*W_b into b.part[4*chunk .. +3].
 void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
                        ShellDev* shells, double* dens);
+// a, b, n of the chunks' triangles; triangle t's component k lands at
+// [k * out_stride + t - out_begin] of ga, gb, gn.
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                         ShellDev* shells, double* ga, double* gb, double* gn);
+                         ShellDev* shells, double* ga, double* gb, double* gn, int out_begin, int out_stride);
 // Output-time diagnostics into out (3T): Gauss curvature det(b)/det(a), mean
 // curvature tr(a^-1 b)/2 and the strain measure sqrt(tr((a^-1 (a-abar))^2)/8)
 // (calc_curvatures.cpp:35-94, calc_strains.cpp:35-78, in invariant form).

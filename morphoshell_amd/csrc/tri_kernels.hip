@@ -79,6 +79,8 @@ struct TriKArgs {
     double* ga;
     double* gb;
     double* gn;
+    int obegin;   // geometry outputs: triangle t at [k * ostride + t - obegin]
+    int ostride;
 };
 
 // ------------------------------------------------------------------ forces
@@ -148,10 +150,11 @@ __global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
             double b[3];
             for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
             if (MODE == TRI_GEOMETRY) {
+                const size_t o = (size_t)(t - k.obegin);
                 for (int kk = 0; kk < 3; ++kk) {
-                    k.ga[kk * T + t] = a[kk];
-                    k.gb[kk * T + t] = b[kk];
-                    k.gn[kk * T + t] = n[kk];
+                    k.ga[kk * (size_t)k.ostride + o] = a[kk];
+                    k.gb[kk * (size_t)k.ostride + o] = b[kk];
+                    k.gn[kk * (size_t)k.ostride + o] = n[kk];
                 }
             } else if (MODE == TRI_DIAGNOSTICS) {
                 // the shape operator a^-1 b and the strain a^-1 (a - abar):
@@ -214,6 +217,8 @@ TriKArgs make_args(const TriData& td, const Buffers& b, const Chunk* chunks, int
     k.nchunks = nchunks;
     k.shells = shells;
     k.part = b.part;
+    k.obegin = 0;
+    k.ostride = td.T;
     return k;
 }
 
@@ -257,11 +262,13 @@ void launch_tri_diagnostics(const Launch& L, const TriData& td, const Buffers& b
 }
 
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                         ShellDev* shells, double* ga, double* gb, double* gn) {
+                         ShellDev* shells, double* ga, double* gb, double* gn, int out_begin, int out_stride) {
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.ga = ga;
     k.gb = gb;
     k.gn = gn;
+    k.obegin = out_begin;
+    k.ostride = out_stride;
     hipLaunchKernelGGL(tri_output_kernel<TRI_GEOMETRY>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 

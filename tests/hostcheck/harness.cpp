@@ -271,6 +271,28 @@ void scenario_zero_dial_and_relax() {
     std::printf("zero-dial and relax ok\n");
 }
 
+// The allocation-free geometry call: no hipMalloc on its first call or
+// later ones, its launch writes inside the scratch array, a second call with
+// a smaller shell follows a larger one, and a shell over the limit is refused
+// without touching the device.
+void scenario_noalloc() {
+    std::vector<Shell> sh{make_shell(40, 9), make_shell(14, 10)};
+    std::vector<ms_shell_desc> d{sh[0].desc(), sh[1].desc()};
+    ms_ctx* c = nullptr;
+    OK(ms_create(d.data(), 2, MS_FP64, &c));
+    set_states(c, sh);
+    for (int s : {0, 1, 0}) {
+        std::vector<double> a(3 * (size_t)sh[s].T), b(a.size()), n(a.size());
+        const long long before = mock::malloc_calls();
+        OK(ms_calc_geometry_noalloc(c, s, a.data(), b.data(), n.data()));
+        REQUIRE(mock::malloc_calls() == before, "ms_calc_geometry_noalloc called hipMalloc");
+        OK(ms_calc_geometry_noalloc(c, s, nullptr, b.data(), nullptr));
+        REQUIRE(mock::malloc_calls() == before, "ms_calc_geometry_noalloc called hipMalloc");
+    }
+    ms_destroy(c);
+    std::printf("allocation-free geometry ok\n");
+}
+
 // Every hipMalloc that a tuning change or a dial-geometry update makes is
 // failed in turn; the call must report the error and leave a context whose
 // later steps touch only live buffers (the replayed graph re-checks them).
@@ -322,6 +344,7 @@ int main() {
     scenario_batch_halts();
     scenario_zero_dial_and_relax();
     scenario_alloc_failures();
+    scenario_noalloc();
     REQUIRE(mock::live_allocations() == 0, "%lld device allocations leaked", mock::live_allocations());
     std::printf("HOSTCHECK OK\n");
     return 0;

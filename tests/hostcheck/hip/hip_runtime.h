@@ -15,6 +15,7 @@
 #define __device__
 #define __global__
 #define __forceinline__ inline
+#define __align__(n) alignas(n)
 
 typedef int hipError_t;
 enum {
@@ -76,6 +77,10 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int dev);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int dev);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetLastError();
+// A __device__ variable is a host array here; its address is registered as
+// a device block (it lives until the process ends).
+#define HIP_SYMBOL(x) mock::symbol(&(x), sizeof(x))
+hipError_t hipGetSymbolAddress(void** p, const void* symbol);
 const char* hipGetErrorString(hipError_t e);
 
 // ---- checks the mock kernels use (mock_runtime.cpp)
@@ -90,4 +95,7 @@ void launch(hipStream_t s, const char* name, std::function<void()> body);
 // (n < 0: never fail).
 void fail_malloc_after(long long n);
 long long live_allocations();
+// hipMalloc calls so far (failed ones included).
+long long malloc_calls();
+const void* symbol(const void* p, size_t bytes);
 }  // namespace mock

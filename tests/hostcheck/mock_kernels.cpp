@@ -167,12 +167,17 @@ void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, con
 }
 
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
-                         ShellDev* shells, double* ga, double* gb, double* gn) {
+                         ShellDev* shells, double* ga, double* gb, double* gn, int out_begin, int out_stride) {
     mock::launch(L.stream, "tri_geometry", [=] {
-        tri_output("tri_geometry", L, td, b, chunks, nchunks, shells, false);
-        dev(ga, 3LL * td.T, WHAT("tri_geometry a", ga));
-        dev(gb, 3LL * td.T, WHAT("tri_geometry b", gb));
-        dev(gn, 3LL * td.T, WHAT("tri_geometry n", gn));
+        const char* w = "tri_geometry";
+        tri_output(w, L, td, b, chunks, nchunks, shells, false);
+        // every chunk's triangles land inside [out_begin, out_begin + out_stride)
+        for (int i = 0; i < nchunks; ++i)
+            if (chunks[i].x < out_begin || chunks[i].y > out_begin + out_stride)
+                bad(w, "chunk outside the output range", chunks[i].x, chunks[i].y);
+        dev(ga, 3LL * out_stride, WHAT(w, ga));
+        dev(gb, 3LL * out_stride, WHAT(w, gb));
+        dev(gn, 3LL * out_stride, WHAT(w, gn));
     });
 }
 

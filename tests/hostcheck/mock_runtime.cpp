@@ -33,8 +33,28 @@ std::map<uintptr_t, size_t>& registry() {
     static std::map<uintptr_t, size_t> r;
     return r;
 }
+std::map<uintptr_t, size_t>& symbols() {
+    static std::map<uintptr_t, size_t> r;
+    return r;
+}
 long long g_fail_after = -1;
+long long g_malloc_calls = 0;
 int g_capturing = 0;
+
+// the live block (heap allocation or device variable) holding address a, or end
+bool find_block(uintptr_t a, uintptr_t& base, size_t& size) {
+    for (auto* r : {&registry(), &symbols()}) {
+        auto it = r->upper_bound(a);
+        if (it == r->begin()) continue;
+        --it;
+        if (a >= it->first && a <= it->first + it->second) {
+            base = it->first;
+            size = it->second;
+            return true;
+        }
+    }
+    return false;
+}
 
 [[noreturn]] void die(const char* what, const char* msg) {
     std::fprintf(stderr, "MOCK-HIP ERROR in %s: %s\n", what, msg);
@@ -56,16 +76,20 @@ void check_dev(const void* p, size_t bytes, const char* what, bool allow_null) {
         die(what, "null device pointer");
     }
     const uintptr_t a = (uintptr_t)p;
-    auto& r = registry();
-    auto it = r.upper_bound(a);
-    if (it == r.begin()) die(what, "pointer is not inside a live device allocation");
-    --it;
-    if (a < it->first || a + bytes > it->first + it->second) {
+    uintptr_t base = 0;
+    size_t size = 0;
+    if (!find_block(a, base, size)) die(what, "pointer is not inside a live device allocation");
+    if (a + bytes > base + size) {
         char m[256];
         std::snprintf(m, sizeof(m), "range [%p, +%zu) outside the live allocation [%p, +%zu)", p, bytes,
-                      (void*)it->first, it->second);
+                      (void*)base, size);
         die(what, m);
     }
+}
+
+const void* symbol(const void* p, size_t bytes) {
+    symbols()[(uintptr_t)p] = bytes;
+    return p;
 }
 
 void launch(hipStream_t s, const char* name, std::function<void()> body) {
@@ -78,10 +102,12 @@ void launch(hipStream_t s, const char* name, std::function<void()> body) {
 
 void fail_malloc_after(long long n) { g_fail_after = n; }
 long long live_allocations() { return (long long)registry().size(); }
+long long malloc_calls() { return g_malloc_calls; }
 }  // namespace mock
 
 hipError_t hipMalloc(void** p, size_t bytes) {
     no_capture("hipMalloc");
+    ++g_malloc_calls;
     *p = nullptr;
     if (g_fail_after == 0) {
         g_fail_after = -1;
@@ -205,6 +231,11 @@ hipError_t hipEventSynchronize(hipEvent_t e) {
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     if (!a || !b || !a->recorded || !b->recorded) die("hipEventElapsedTime", "event never recorded");
     *ms = 0.001f;
+    return hipSuccess;
+}
+hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    if (!symbols().count((uintptr_t)symbol)) die("hipGetSymbolAddress", "not a registered device variable");
+    *p = const_cast<void*>(symbol);
     return hipSuccess;
 }
 hipError_t hipGetDeviceCount(int* n) {
