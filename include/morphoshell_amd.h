@@ -265,6 +265,21 @@ int ms_calc_energy_densities(ms_ctx* ctx, int32_t shell, double* densities);
  * (calc_strains.cpp:35-78), both evaluated in their invariant forms
  * det(b)/det(a), tr(a^-1 b)/2 and sqrt(tr((a^-1 (a - abar))^2)/8). */
 int ms_calc_diagnostics(ms_ctx* ctx, int32_t shell, double* gauss_curv, double* mean_curv, double* strain);
+/* calc_stresses (calc_stresses.cpp:45-179) on the current state: per
+ * triangle the tensor (A/J) sum_alpha dW/dR_alpha (x) R_alpha (the reference
+ * area A times the membrane Cauchy stress; W the energy density per
+ * reference area, J the area ratio), its symmetric part read from the lower
+ * triangle, and of its eigenpairs the two the reference keeps: 8T SoA
+ * (eigval_1.., eigvec_1 x.., y.., z.., eigval_2.., eigvec_2 x.., y.., z..),
+ * eigenvectors with their largest-magnitude component positive.
+ * write_output.cpp:136-156 divides the eigenvalues by the reference shear
+ * modulus times thickness of each triangle. */
+int ms_calc_stresses(ms_ctx* ctx, int32_t shell, double* stresses);
+/* calc_angle_deficits (calc_angle_deficits.cpp:40-78) on the current state:
+ * per node 2 pi (pi where boundary[n] != 0; N flags of the shell, e.g. from
+ * ms_mesh_info) minus the corner angles of its triangles, added in ascending
+ * triangle order as the reference's serial loop does.  deficits: N. */
+int ms_calc_angle_deficits(ms_ctx* ctx, int32_t shell, const uint8_t* boundary, double* deficits);
 /* Current force buffer of one shell (3N dofs-SoA). */
 int ms_get_forces(ms_ctx* ctx, int32_t shell, double* forces);
 int ms_set_forces(ms_ctx* ctx, int32_t shell, const double* forces);

@@ -45,6 +45,8 @@ struct ms_ctx {
     int* csr_ent = nullptr;
     Buffers b{};
     double* scratch3 = nullptr;  // 9T (geometry outputs) / 2T densities
+    double* node_out = nullptr;  // N (angle deficits), allocated on first use
+    unsigned char* node_flags = nullptr;  // N boundary flags of the angle deficits
     double* red_out = nullptr;   // per-shell reduction outputs (3 per shell)
 
     // launch shapes
@@ -893,6 +895,8 @@ extern "C" void ms_destroy(ms_ctx* c) {
     if (c->d_act_tri) hipFree(c->d_act_tri);
     if (c->d_act_node) hipFree(c->d_act_node);
     if (c->scratch3) hipFree(c->scratch3);
+    if (c->node_out) hipFree(c->node_out);
+    if (c->node_flags) hipFree(c->node_flags);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1235,6 +1239,40 @@ extern "C" int ms_calc_diagnostics(ms_ctx* c, int32_t s, double* gauss, double* 
         if (outs[q])
             HIPC(c, hipMemcpy(outs[q], c->scratch3 + q * T + S.tri_begin, sizeof(double) * S.num_tris,
                               hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+
+extern "C" int ms_calc_stresses(ms_ctx* c, int32_t s, double* stresses) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!stresses) return fail(c, MS_ERR_ARG, "ms_calc_stresses: null output");
+    if (int r = ensure_scratch(c)) return r;  // 9T doubles, 8T used
+    const size_t T = c->T;
+    launch_tri_stresses(tri_launch(c), tri_data(c), c->b, c->d_tri_chunks, c->n_tri_chunks, c->d_shells,
+                        c->scratch3);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const ShellDev& S = c->hs[s];
+    for (int k = 0; k < 8; ++k)
+        HIPC(c, hipMemcpy(stresses + (size_t)k * S.num_tris, c->scratch3 + k * T + S.tri_begin,
+                          sizeof(double) * S.num_tris, hipMemcpyDeviceToHost));
+    return MS_OK;
+}
+
+extern "C" int ms_calc_angle_deficits(ms_ctx* c, int32_t s, const uint8_t* boundary, double* deficits) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!boundary || !deficits) return fail(c, MS_ERR_ARG, "ms_calc_angle_deficits: null argument");
+    if (!c->node_out) HIPC(c, hipMalloc((void**)&c->node_out, sizeof(double) * (size_t)c->N));
+    if (!c->node_flags) HIPC(c, hipMalloc((void**)&c->node_flags, (size_t)c->N));
+    const ShellDev& S = c->hs[s];
+    std::vector<unsigned char> fl(S.num_nodes);
+    for (int n = 0; n < S.num_nodes; ++n) fl[n] = boundary[n] ? 1 : 0;
+    HIPC(c, hipMemcpy(c->node_flags + S.node_begin, fl.data(), fl.size(), hipMemcpyHostToDevice));
+    const Launch L{grid_for(c, S.node_chunk_count, c->tune.node_grid_per_cu), c->tune.node_block, c->stream};
+    launch_node_angle_deficits(L, node_data(c), c->b, c->d_node_chunks + S.node_chunk_begin, S.node_chunk_count,
+                               c->d_shells, c->pid, c->T, c->node_flags, c->node_out);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipMemcpy(deficits, c->node_out + S.node_begin, sizeof(double) * S.num_nodes, hipMemcpyDeviceToHost));
     return MS_OK;
 }
 

@@ -47,7 +47,7 @@ struct Buffers {
     long long* bad_dof;       // per shell, ops-path blow-up scan
 };
 
-enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2, TRI_DIAGNOSTICS = 3 };
+enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2, TRI_DIAGNOSTICS = 3, TRI_STRESS = 4 };
 
 struct Launch {
     int grid;
@@ -66,6 +66,11 @@ void launch_tri_energy(const Launch& L, const TriData& td, const Buffers& b, con
 // [k * out_stride + t - out_begin] of ga, gb, gn.
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
                          ShellDev* shells, double* ga, double* gb, double* gn, int out_begin, int out_stride);
+// Output-time Cauchy stress (calc_stresses.cpp:45-179) into out (8T, SoA):
+// eigenvalue 1, eigenvector 1 (3), eigenvalue 2, eigenvector 2 (3), the
+// eigenvalues still carrying the reference area (stress_math.h).
+void launch_tri_stresses(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, double* out);
 // Output-time diagnostics into out (3T): Gauss curvature det(b)/det(a), mean
 // curvature tr(a^-1 b)/2 and the strain measure sqrt(tr((a^-1 (a-abar))^2)/8)
 // (calc_curvatures.cpp:35-94, calc_strains.cpp:35-78, in invariant form).
@@ -98,6 +103,11 @@ void launch_node_equil(const Launch& L, const NodeData& nd, const Buffers& b, co
 // Kinetic energy partials (double-double) into b.part[4*chunk + 0..1].
 void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                          ShellDev* shells);
+// Angle deficits of the chunks' nodes into out[n] (global node index);
+// boundary[n] = 1 on boundary nodes (calc_angle_deficits.cpp:40-78).
+void launch_node_angle_deficits(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks,
+                                int nchunks, ShellDev* shells, const int* pid, int T, const unsigned char* boundary,
+                                double* out);
 // Per-shell compensated combine of chunk partials: out[shell*ncomp + c] for
 // components c < ncomp stored as (hi, lo) pairs at part[4*chunk + 2c].
 void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* shells, int num_shells, bool tri_chunks,

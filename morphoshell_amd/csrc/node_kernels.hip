@@ -452,6 +452,52 @@ __global__ __launch_bounds__(512) void node_kinetic_kernel(NodeKArgs k) {
     }
 }
 
+// Angle deficits (calc_angle_deficits.cpp:40-78): per node 2 pi (pi on the
+// boundary) minus the corner angles of its triangles, added in ascending
+// triangle order -- the order of the reference's serial loop -- from the
+// vertex slots (0-2) of the patch-incidence CSR.  The corner at vertex k is
+// acos(-(e_{k-1} . e_k) / (|e_{k-1}| |e_k|)) with e_0 = P_1 - P_0,
+// e_1 = P_2 - P_1, e_2 = -e_1 - e_0.
+__global__ __launch_bounds__(512) void node_angle_deficit_kernel(NodeKArgs k, const int* pid, int T,
+                                                                 const unsigned char* boundary, double* out) {
+#pragma clang fp contract(off)
+    const int N = k.nd.N;
+    const double pi = 3.14159265358979323846;
+    for (int vi = blockIdx.x; vi < k.nchunks; vi += gridDim.x) {
+        const Chunk ch = k.chunks[xcd_remap(vi, k.nchunks)];
+        const ShellDev& sh = k.shells[ch.z];
+        const int n = ch.x + threadIdx.x;
+        if (n >= ch.y) continue;
+        const double* X = k.b.x[sh.par];
+        double d = boundary[n] ? pi : 2.0 * pi;
+        for (long long e = k.nd.csr_off[n]; e < k.nd.csr_off[n + 1]; ++e) {
+            const int ent = k.nd.csr_ent[e];
+            const int slot = ent % 6;
+            if (slot >= 3) continue;
+            const int t = ent / 6;
+            double P[3][3];
+            for (int v = 0; v < 3; ++v) {
+                const int q = pid[(size_t)v * T + t];
+                P[v][0] = X[q];
+                P[v][1] = X[N + q];
+                P[v][2] = X[2 * (size_t)N + q];
+            }
+            double s[3][3];
+            for (int c = 0; c < 3; ++c) {
+                s[0][c] = P[1][c] - P[0][c];
+                s[1][c] = P[2][c] - P[1][c];
+                s[2][c] = -s[1][c] - s[0][c];
+            }
+            const int km = (slot + 2) % 3;  // the side arriving at this corner
+            const double dot = s[km][0] * s[slot][0] + s[km][1] * s[slot][1] + s[km][2] * s[slot][2];
+            const double lm = sqrt(s[km][0] * s[km][0] + s[km][1] * s[km][1] + s[km][2] * s[km][2]);
+            const double lk = sqrt(s[slot][0] * s[slot][0] + s[slot][1] * s[slot][1] + s[slot][2] * s[slot][2]);
+            d += -acos(-dot / (lm * lk));
+        }
+        out[n] = d;
+    }
+}
+
 __global__ void reduce_partials_kernel(const double* part, const ShellDev* shells, int num_shells, int tri_chunks,
                                        int ncomp, double* out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,6 +608,13 @@ void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, 
                          ShellDev* shells) {
     NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
     hipLaunchKernelGGL(node_kinetic_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_node_angle_deficits(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks,
+                                int nchunks, ShellDev* shells, const int* pid, int T, const unsigned char* boundary,
+                                double* out) {
+    NodeKArgs k = make_args(nd, b, chunks, nchunks, shells, 0, nullptr);
+    hipLaunchKernelGGL(node_angle_deficit_kernel, dim3(L.grid), dim3(L.block), 0, L.stream, k, pid, T, boundary, out);
 }
 
 void launch_reduce_partials(hipStream_t s, const double* part, const ShellDev* shells, int num_shells, bool tri_chunks,

@@ -11,6 +11,7 @@
 #include <math.h>
 
 #include "kernels.h"
+#include "stress_math.h"
 #include "tri_math.h"
 
 namespace ms {
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
             const double m0 = Rx[1] * Ry[2] - Rx[2] * Ry[1];
             const double m1 = Rx[2] * Ry[0] - Rx[0] * Ry[2];
             const double m2 = Rx[0] * Ry[1] - Rx[1] * Ry[0];
-            const double imn = 1.0 / sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+            const double mn = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+            const double imn = 1.0 / mn;
             const double n[3] = {m0 * imn, m1 * imn, m2 * imn};
             double b[3];
             for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
@@ -156,6 +158,13 @@ __global__ __launch_bounds__(256) void tri_output_kernel(TriKArgs k) {
                     k.gb[kk * (size_t)k.ostride + o] = b[kk];
                     k.gn[kk * (size_t)k.ostride + o] = n[kk];
                 }
+            } else if (MODE == TRI_STRESS) {
+                double ab[3], bb[3], S[6], lam[3], V[3][3], o[8];
+                targets(td, tri_params(sh), (unsigned)t, ab, bb);
+                cauchy_lower(sh.nu, sh.gamma, td.h[t], td.mu[t], td.area[t], ab, bb, Rx, Ry, R2, n, b, mn, S);
+                sym3_eigen(S, lam, V);
+                stress_pack(lam, V, o);
+                for (int kk = 0; kk < 8; ++kk) k.ga[kk * (size_t)T + t] = o[kk];
             } else if (MODE == TRI_DIAGNOSTICS) {
                 // the shape operator a^-1 b and the strain a^-1 (a - abar):
                 // invariants of the reference's QR and pseudo-inverse forms
@@ -259,6 +268,13 @@ void launch_tri_diagnostics(const Launch& L, const TriData& td, const Buffers& b
     TriKArgs k = make_args(td, b, chunks, nchunks, shells);
     k.ga = out;
     hipLaunchKernelGGL(tri_output_kernel<TRI_DIAGNOSTICS>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
+}
+
+void launch_tri_stresses(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, double* out) {
+    TriKArgs k = make_args(td, b, chunks, nchunks, shells);
+    k.ga = out;
+    hipLaunchKernelGGL(tri_output_kernel<TRI_STRESS>, dim3(L.grid), dim3(L.block), 0, L.stream, k);
 }
 
 void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,

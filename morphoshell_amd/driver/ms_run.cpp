@@ -17,9 +17,7 @@
 //
 // Differences, each deliberate: the run stops at the first equilibrium at
 // dial factor 1 (the reference's loop has no end unless glass slides are
-// on, main.cpp:493); glass slides are refused; the output omits the Cauchy
-// stress field (its definition exists only as generated code in
-// calc_stresses.cpp); equilibrium checks are not logged one by one.
+// on, main.cpp:493); glass slides are refused.
 
 #include <unistd.h>
 
@@ -128,7 +126,7 @@ struct OutputJob {
     int64_t stepcount;
     double dial_factor;
     int status;
-    std::vector<double> dofs, gauss, mean, strain, dens;
+    std::vector<double> dofs, gauss, mean, strain, dens, stresses, deficit;
 };
 
 // Single background thread writing stepcount_<n>_output.vtk files (and the
@@ -244,14 +242,12 @@ ms_shell_params params_for(const Run& r, const ms_scales& s, bool gradient_desce
     return p;
 }
 
-// Formats and writes one output (writer thread): angle deficits, the VTK file
-// and the aux_output.txt row (main.cpp:409-437, write_output.cpp).
+// Formats and writes one output (writer thread): the VTK file and the
+// aux_output.txt row (main.cpp:409-437, write_output.cpp).
 void write_job(const Run& r, const OutputJob& j) {
     const int T = r.mesh.num_tris;
-    std::vector<double> deficit;
-    angle_deficits(r.mesh, r.boundary, j.dofs.data(), deficit);
-    OutputFields f{j.dofs.data(), j.gauss.data(), j.mean.data(), j.dens.data(), j.dens.data() + T, j.strain.data(),
-                   deficit.data()};
+    OutputFields f{j.dofs.data(),   j.gauss.data(),    j.mean.data(),    j.dens.data(), j.dens.data() + T,
+                   j.strain.data(), j.stresses.data(), j.deficit.data()};
     write_output_vtk(r.out_dir, j.stepcount, j.dial_factor, r.mesh, f);
     std::ofstream aux(r.aux_path, std::ios::app);
     if (j.stepcount == 0)
@@ -262,16 +258,34 @@ void write_job(const Run& r, const OutputJob& j) {
 }
 
 // The device part of an output (state, curvatures, strain, energy
-// densities), copied to the host and handed to the writer thread.
+// densities, Cauchy stress eigenpairs, angle deficits), copied to the host
+// and handed to the writer thread.
 void write_output(Run& r, int64_t stepcount, double dial_factor, int status) {
     const int N = r.mesh.num_nodes, T = r.mesh.num_tris;
-    OutputJob j{stepcount, dial_factor, status, std::vector<double>(3 * (size_t)N), std::vector<double>(T),
-                std::vector<double>(T), std::vector<double>(T), std::vector<double>(2 * (size_t)T)};
+    OutputJob j{stepcount,
+                dial_factor,
+                status,
+                std::vector<double>(3 * (size_t)N),
+                std::vector<double>(T),
+                std::vector<double>(T),
+                std::vector<double>(T),
+                std::vector<double>(2 * (size_t)T),
+                std::vector<double>(8 * (size_t)T),
+                std::vector<double>(N)};
     check(ms_get_state(r.ctx, 0, j.dofs.data(), nullptr), "ms_get_state");
     check(ms_calc_diagnostics(r.ctx, 0, j.gauss.data(), j.mean.data(), j.strain.data()), "ms_calc_diagnostics");
     check(ms_calc_energy_densities(r.ctx, 0, j.dens.data()), "ms_calc_energy_densities");
+    check(ms_calc_stresses(r.ctx, 0, j.stresses.data()), "ms_calc_stresses");
+    check(ms_calc_angle_deficits(r.ctx, 0, r.boundary.data(), j.deficit.data()), "ms_calc_angle_deficits");
     const double scale = r.sc.char_stretch_energy_density_scale;
     for (auto& d : j.dens) d /= scale;
+    // write_output.cpp:139-152: eigenvalues over the reference shear modulus
+    // times the reference thickness of each triangle
+    for (int t = 0; t < T; ++t) {
+        const double mh = r.mesh.shear_modulus[t] * r.mesh.thickness[t];
+        j.stresses[t] /= mh;
+        j.stresses[4 * (size_t)T + t] /= mh;
+    }
     r.writer->submit(std::move(j));
 }
 

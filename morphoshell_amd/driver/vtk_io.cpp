@@ -320,36 +320,22 @@ void write_output_vtk(const std::string& dir, int64_t stepcount, double dial_fac
     scalars("dimless_stretch_energy_density", f.stretch_density, T);
     scalars("dimless_bend_energy_density", f.bend_density, T);
     scalars("strain_measure", f.strain, T);
+    // write_output.cpp:136-156: the two kept eigenpairs of the Cauchy stress
+    out << "FIELD cauchy_stress_info 4\n";
+    auto field = [&](const char* name, int ncomp, int first) {
+        out << name << " " << ncomp << " " << T << " double\n";
+        for (int t = 0; t < T; ++t) {
+            for (int c = 0; c < ncomp; ++c) out << (c ? " " : "") << f.stresses[(size_t)(first + c) * T + t];
+            out << "\n";
+        }
+    };
+    field("dimless_cauchy_stress_eigval_1", 1, 0);
+    field("cauchy_stress_eigvec_1", 3, 1);
+    field("dimless_cauchy_stress_eigval_2", 1, 4);
+    field("cauchy_stress_eigvec_2", 3, 5);
     out << "POINT_DATA " << N << "\n";
     scalars("angleDeficit", f.angle_deficit, N);
     if (!out) throw std::runtime_error("Error: Problem writing vtk output file.");
-}
-
-void angle_deficits(const RefMesh& m, const std::vector<uint8_t>& boundary, const double* dofs,
-                    std::vector<double>& out) {
-    const double pi = 3.14159265358979323846;
-    const size_t N = (size_t)m.num_nodes;
-    out.assign(N, 0.0);
-    for (size_t n = 0; n < N; ++n) out[n] = boundary[n] ? pi : 2.0 * pi;
-    for (int t = 0; t < m.num_tris; ++t) {
-        const int32_t* v = &m.tris[3 * (size_t)t];
-        double P[3][3];
-        for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 3; ++c) P[k][c] = dofs[c * N + v[k]];
-        double e[3][3];  // e_0 = P_1 - P_0, e_1 = P_2 - P_1, e_2 = -e_1 - e_0 (closing the loop)
-        for (int c = 0; c < 3; ++c) {
-            e[0][c] = P[1][c] - P[0][c];
-            e[1][c] = P[2][c] - P[1][c];
-            e[2][c] = -e[1][c] - e[0][c];
-        }
-        double len[3];
-        for (int k = 0; k < 3; ++k) len[k] = std::sqrt(e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2]);
-        for (int k = 0; k < 3; ++k) {
-            const int km = (k + 2) % 3;  // the edge arriving at corner k
-            const double dot = e[km][0] * e[k][0] + e[km][1] * e[k][1] + e[km][2] * e[k][2];
-            out[v[k]] += -std::acos(-dot / (len[km] * len[k]));
-        }
-    }
 }
 
 }  // namespace msrun

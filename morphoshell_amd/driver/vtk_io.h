@@ -50,16 +50,12 @@ struct OutputFields {
     const double* stretch_density;   // T, already divided by the energy-density scale
     const double* bend_density;      // T, likewise
     const double* strain;            // T
+    const double* stresses;          // 8T SoA: eigval_1 (over mu h), eigvec_1 x, y, z, eigval_2, eigvec_2 x, y, z
     const double* angle_deficit;     // N
 };
 
 void write_output_vtk(const std::string& dir, int64_t stepcount, double dial_factor, const RefMesh& m,
                       const OutputFields& f);
 
-// Angle deficits (calc_angle_deficits.cpp:44-78): 2 pi (pi on the boundary)
-// minus the corner angles of the incident triangles, accumulated in
-// triangle order.
-void angle_deficits(const RefMesh& m, const std::vector<uint8_t>& boundary, const double* dofs,
-                    std::vector<double>& out);
 
 }  // namespace msrun

@@ -25,7 +25,7 @@ EXPORTED = [
     "ms_set_dial_clock", "ms_get_dial_factor", "ms_set_shell_params", "ms_set_zero_dial_geometry",
     "ms_set_precision", "ms_get_precision", "ms_calc_deformation_forces", "ms_calc_non_deformation_forces",
     "ms_impose_constraints", "ms_advance_dynamics", "ms_is_at_equilibrium", "ms_calc_energies",
-    "ms_calc_geometry", "ms_calc_geometry_noalloc", "ms_calc_energy_densities", "ms_calc_diagnostics", "ms_get_forces", "ms_set_forces", "ms_step",
+    "ms_calc_geometry", "ms_calc_geometry_noalloc", "ms_calc_stresses", "ms_calc_angle_deficits", "ms_calc_energy_densities", "ms_calc_diagnostics", "ms_get_forces", "ms_set_forces", "ms_step",
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
     "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_calibration_read",
     "ms_device_info", "ms_relax_setup", "ms_relax_run", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
@@ -177,6 +177,8 @@ def lib() -> C.CDLL:
         "ms_calc_energies": (C.c_int, [P, C.c_int32, _dp]),
         "ms_calc_geometry": (C.c_int, [P, C.c_int32, _dp, _dp, _dp]),
         "ms_calc_geometry_noalloc": (C.c_int, [P, C.c_int32, _dp, _dp, _dp]),
+        "ms_calc_stresses": (C.c_int, [P, C.c_int32, _dp]),
+        "ms_calc_angle_deficits": (C.c_int, [P, C.c_int32, C.c_void_p, _dp]),
         "ms_calc_energy_densities": (C.c_int, [P, C.c_int32, _dp]),
         "ms_calc_diagnostics": (C.c_int, [P, C.c_int32, _dp, _dp, _dp]),
         "ms_get_forces": (C.c_int, [P, C.c_int32, _dp]),

@@ -468,6 +468,25 @@ class Context:
         self._ck(self.L.ms_calc_diagnostics(self.h, shell, nat.dptr(g), nat.dptr(m), nat.dptr(s)))
         return {"gauss_curv": g, "mean_curv": m, "strain": s}
 
+    def calc_stresses(self, shell: int = 0) -> np.ndarray:
+        """calc_stresses (calc_stresses.cpp:45-179): (8, T) per triangle --
+        eigenvalue 1, eigenvector 1 (3), eigenvalue 2, eigenvector 2 (3) of
+        the reference-area-weighted Cauchy stress; write_output divides the
+        eigenvalues by the reference shear modulus times thickness."""
+        T = self._shell(shell).T
+        out = np.empty(8 * T)
+        self._ck(self.L.ms_calc_stresses(self.h, shell, nat.dptr(out)))
+        return out.reshape(8, T)
+
+    def calc_angle_deficits(self, shell: int = 0) -> np.ndarray:
+        """calc_angle_deficits (calc_angle_deficits.cpp:40-78): per node 2 pi
+        (pi on the boundary) minus its triangles' corner angles."""
+        sh = self._shell(shell)
+        bnd = np.ascontiguousarray(sh.info["boundary_nodes"], dtype=np.uint8)
+        out = np.empty(sh.N)
+        self._ck(self.L.ms_calc_angle_deficits(self.h, shell, bnd.ctypes.data, nat.dptr(out)))
+        return out
+
     # -- fused path
     def step(self, nsteps: int, equil_every: int = 0, force_tol: float = 1e-6, speed_tol: float = 1e-5,
              halt_on_equilibrium: bool = False, raise_on_blowup: bool = True) -> list[dict]:

@@ -169,6 +169,36 @@ class System:
         lib().orc_geometry(self.h, _d(np.ascontiguousarray(dofs)), _d(a), _d(b), _d(n))
         return a.reshape(3, -1), b.reshape(3, -1), n.reshape(3, -1)
 
+    def stresses(self, dofs):
+        """calc_stresses (calc_stresses.cpp:45-179) restated from the
+        oracle's own gradient g = d(A (W_s + W_b))/dq (dE_dq, pinned by
+        finite differences): per triangle sigma_ij = (1/J) sum_alpha
+        g[R^i_alpha] R^j_alpha with J = |R_x x R_y|; the symmetric matrix of
+        its lower triangle (what Eigen's SelfAdjointEigenSolver reads,
+        calc_stresses.cpp:162) diagonalised by LAPACK (numpy eigh, lower
+        triangle, ascending); the first eigenvalue of least magnitude is
+        dropped and the next two kept in cyclic order (:165-176).  Returns
+        (8, T): eigval_1, eigvec_1 (3), eigval_2, eigvec_2 (3), each
+        eigenvector with its largest-magnitude component positive."""
+        q = self.surface_derivs(dofs).reshape(self.T, 15)
+        g = self.dE_dq(q.reshape(-1)).reshape(self.T, 15)
+        Rx, Ry = q[:, [0, 5, 10]], q[:, [1, 6, 11]]
+        gx, gy = g[:, [0, 5, 10]], g[:, [1, 6, 11]]
+        J = np.linalg.norm(np.cross(Rx, Ry), axis=1)
+        sig = (gx[:, :, None] * Rx[:, None, :] + gy[:, :, None] * Ry[:, None, :]) / J[:, None, None]
+        lam, V = np.linalg.eigh(sig, UPLO="L")
+        m = np.argmin(np.abs(lam), axis=1)  # first of least magnitude
+        out = np.empty((8, self.T))
+        rows = np.arange(self.T)
+        for r in range(2):
+            k = (m + 1 + r) % 3
+            vec = V[rows, :, k]
+            big = np.argmax(np.abs(vec), axis=1)
+            sgn = np.where(vec[rows, big] < 0.0, -1.0, 1.0)
+            out[4 * r] = lam[rows, k]
+            out[4 * r + 1:4 * r + 4] = (vec * sgn[:, None]).T
+        return out
+
     def energy_densities(self, dofs):
         d = np.zeros(2 * self.T)
         lib().orc_energy_densities(self.h, _d(np.ascontiguousarray(dofs)), _d(d))

@@ -167,6 +167,10 @@ void exercise_ops(ms_ctx* c, const std::vector<Shell>& shells) {
         std::vector<double> a(3 * (size_t)sh.T), b(3 * (size_t)sh.T), n(3 * (size_t)sh.T), f(3 * (size_t)sh.N);
         OK(ms_calc_geometry(c, s, a.data(), b.data(), n.data()));
         OK(ms_calc_diagnostics(c, s, a.data(), b.data(), n.data()));
+        std::vector<double> st(8 * (size_t)sh.T), ad(sh.N);
+        OK(ms_calc_stresses(c, s, st.data()));
+        std::vector<uint8_t> bnd(sh.N, 0);
+        OK(ms_calc_angle_deficits(c, s, bnd.data(), ad.data()));
         OK(ms_calc_energy_densities(c, s, a.data()));
         OK(ms_get_forces(c, s, f.data()));
         OK(ms_set_forces(c, s, f.data()));

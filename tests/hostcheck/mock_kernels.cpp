@@ -181,6 +181,14 @@ void launch_tri_geometry(const Launch& L, const TriData& td, const Buffers& b, c
     });
 }
 
+void launch_tri_stresses(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
+                         ShellDev* shells, double* out) {
+    mock::launch(L.stream, "tri_stresses", [=] {
+        tri_output("tri_stresses", L, td, b, chunks, nchunks, shells, false);
+        dev(out, 8LL * td.T, "tri_stresses out");
+    });
+}
+
 void launch_tri_diagnostics(const Launch& L, const TriData& td, const Buffers& b, const Chunk* chunks, int nchunks,
                             ShellDev* shells, double* out) {
     mock::launch(L.stream, "tri_diagnostics", [=] {
@@ -306,6 +314,20 @@ void launch_node_kinetic(const Launch& L, const NodeData& nd, const Buffers& b, 
     mock::launch(L.stream, "node_kinetic", [=] {
         node_common("node_kinetic", L, nd, b, chunks, nchunks, shells, 0);
         for (int i = 0; i < nchunks; ++i) dev(b.part + 4LL * i, 2, WHAT("node_kinetic", b.part + 4LL * i));
+    });
+}
+
+void launch_node_angle_deficits(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks,
+                                int nchunks, ShellDev* shells, const int* pid, int T, const unsigned char* boundary,
+                                double* out) {
+    mock::launch(L.stream, "node_angle_deficits", [=] {
+        const char* w = "node_angle_deficits";
+        node_common(w, L, nd, b, chunks, nchunks, shells, 0);
+        dev(pid, 6LL * T, WHAT(w, pid));
+        for (int i = 0; i < nchunks; ++i) {
+            dev(boundary + chunks[i].x, chunks[i].y - chunks[i].x, WHAT(w, boundary));
+            dev(out + chunks[i].x, chunks[i].y - chunks[i].x, WHAT(w, out));
+        }
     });
 }
 

@@ -147,6 +147,20 @@ def test_driver_run_matches_context_relax(tmp_path, spacing):
     lines, x_drv = _read_output(files[-1], m.num_nodes)
     assert lines[1].startswith("dial_factor = 1.00000000000000e+00")
     assert "SCALARS gauss_curv double 1" in lines and "SCALARS angleDeficit double 1" in lines
+    # write_output.cpp:136-156: the Cauchy stress block between the cell
+    # scalars and the point data
+    T, N = m.num_tris, m.num_nodes
+    i = lines.index("FIELD cauchy_stress_info 4")
+    assert lines[i - T - 2] == "SCALARS strain_measure double 1"
+    blocks = {}
+    j = i + 1
+    for name, nc in (("dimless_cauchy_stress_eigval_1", 1), ("cauchy_stress_eigvec_1", 3),
+                     ("dimless_cauchy_stress_eigval_2", 1), ("cauchy_stress_eigvec_2", 3)):
+        assert lines[j] == f"{name} {nc} {T} double"
+        blocks[name] = np.array([[float(v) for v in ln.split()] for ln in lines[j + 1:j + 1 + T]])
+        j += T + 1
+    assert lines[j] == f"POINT_DATA {N}"
+    deficit = np.array([float(v) for v in lines[j + 3:j + 3 + N]])
 
     # the same protocol through the Python context, from the same start
     sh = prepare_shell(m, read_settings(st.read_text()))
@@ -161,9 +175,19 @@ def test_driver_run_matches_context_relax(tmp_path, spacing):
             if rep["status"] != 2:
                 ctx.resume(0)
         assert rep["status"] == 2
+        # the output's stress eigenpairs and angle deficits are the context's at that state
+        S = ctx.calc_stresses(0)
+        d_ctx = ctx.calc_angle_deficits(0)
     assert eq_step == last_step
     # the output prints 15 significant digits
     assert np.allclose(x_drv, x_ctx, rtol=0, atol=1e-13 * np.abs(x_ctx).max() + 1e-300)
+    mh = sh.shear_modulus * sh.thickness
+    for r, (ev, vec) in enumerate((("dimless_cauchy_stress_eigval_1", "cauchy_stress_eigvec_1"),
+                                   ("dimless_cauchy_stress_eigval_2", "cauchy_stress_eigvec_2"))):
+        lam = S[4 * r] / mh
+        assert np.allclose(blocks[ev][:, 0], lam, rtol=1e-13, atol=1e-13 * np.abs(lam).max())
+        assert np.allclose(blocks[vec], S[4 * r + 1:4 * r + 4].T, rtol=1e-13, atol=1e-13)
+    assert np.allclose(deficit, d_ctx, rtol=1e-13, atol=1e-13)
 
 
 @pytest.mark.gpu

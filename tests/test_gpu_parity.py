@@ -859,3 +859,97 @@ def test_halted_shells_fused_active_tiles(precision):
             for i in running:
                 assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
                 assert np.array_equal(a.get_state(i)[1], b.get_state(i)[1]), i
+
+
+def test_cauchy_stress_matches_oracle(sh):
+    """calc_stresses (calc_stresses.cpp:45-179) at a bent and stretched state
+    with bend stiffening active: the device's kept eigenvalues match the
+    oracle's (its own gradient, LAPACK eigensolver) to 1e-10 of the largest,
+    and the eigenvectors agree to 1e-9 (and in sign) wherever the pair is
+    well separated from each other and from the dropped eigenvalue."""
+    x, v = deformed_state(sh, seed=31)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        Sd = ctx.calc_stresses(0)
+    So = orc.System.from_shell(sh).stresses(x)
+    lam_o = So[[0, 4]]
+    scale = np.abs(lam_o).max()
+    err = np.abs(Sd[[0, 4]] - lam_o).max() / scale
+    print(f"stress eigenvalue err {err:.2e} of the largest ({scale:.3e})")
+    assert err <= 1e-10
+    gap = np.minimum(np.abs(lam_o[0] - lam_o[1]), np.minimum(np.abs(lam_o[0]), np.abs(lam_o[1])))
+    ok = gap > 1e-6 * scale
+    assert ok.mean() > 0.9
+    worst = 0.0
+    for r in range(2):
+        vd, vo = Sd[4 * r + 1:4 * r + 4, ok], So[4 * r + 1:4 * r + 4, ok]
+        worst = max(worst, float(np.abs(vd - vo).max()))
+    print(f"eigenvector err {worst:.2e} over {ok.sum()} of {sh.T} triangles")
+    assert worst <= 1e-9
+
+
+def test_cauchy_stress_closed_form_on_the_device():
+    """A flat sheet with abar = I, bbar = 0, stretched by (1.02, 0.97) and
+    rotated: eigenvalues 2 A c_s g0 sx/sy and 2 A c_s g2 sy/sx along the
+    rotated stretch axes (c_s = h mu/(1 - nu), g0 = 2 nu (ex + ey) +
+    2 (1 - nu) ex, ex = sx^2 - 1), the normal's zero dropped."""
+    import math
+
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    m = grf_disc(0.08, seed=32, h=0.02, kappa=0.0)
+    m.abar_info[:] = [1.0, 0.0, 1.0]
+    s = prepare_shell(m, Settings())
+    N = s.N
+    sx, sy = 1.02, 0.97
+    x = s.ref_pos.copy()
+    x[:N] *= sx
+    x[N:2 * N] *= sy
+    x[2 * N:] = 0.0
+    c, si = math.cos(0.5), math.sin(0.5)
+    Q = np.array([[c, 0.0, si], [0.0, 1.0, 0.0], [-si, 0.0, c]])
+    x = (Q @ x.reshape(3, N)).reshape(-1)
+    with _ctx(s) as ctx:
+        ctx.set_state(0, x)
+        S = ctx.calc_stresses(0)
+    nu = s.settings.def_poisson_ratio
+    cs = s.thickness * s.shear_modulus / (1.0 - nu)
+    ex, ey = sx * sx - 1.0, sy * sy - 1.0
+    l_x = 2.0 * s.areas * cs * (2.0 * nu * (ex + ey) + 2.0 * (1.0 - nu) * ex) * sx / sy
+    l_y = 2.0 * s.areas * cs * (2.0 * nu * (ex + ey) + 2.0 * (1.0 - nu) * ey) * sy / sx
+    # ascending eigenvalues are (l_y, 0, l_x) here (l_y < 0 < l_x): the zero is
+    # dropped and the pair kept in cyclic order is (l_x, l_y)
+    assert (l_y < 0).all() and (l_x > 0).all()
+    assert np.abs(S[0] - l_x).max() <= 1e-12 * np.abs(l_x).max()
+    assert np.abs(S[4] - l_y).max() <= 1e-12 * np.abs(l_x).max()
+    assert np.abs(np.abs(Q[:, 0] @ S[1:4]) - 1.0).max() <= 1e-12
+    assert np.abs(np.abs(Q[:, 1] @ S[5:8]) - 1.0).max() <= 1e-12
+
+
+def test_angle_deficits_match_host_and_gauss_bonnet(sh):
+    """calc_angle_deficits (calc_angle_deficits.cpp:40-78): against a numpy
+    restatement summing the corner angles in triangle order (to 1e-13,
+    acos may differ by an ulp between the device and the host libraries),
+    and discrete Gauss-Bonnet: over a disc the deficits sum to 2 pi."""
+    x, v = deformed_state(sh, seed=33)
+    with _ctx(sh) as ctx:
+        ctx.set_state(0, x, v)
+        d = ctx.calc_angle_deficits(0)
+    N = sh.N
+    P = x.reshape(3, N).T
+    tri = sh.mesh.tris
+    ref = np.where(sh.info["boundary_nodes"] != 0, np.pi, 2.0 * np.pi)
+    s0 = P[tri[:, 1]] - P[tri[:, 0]]
+    s1 = P[tri[:, 2]] - P[tri[:, 1]]
+    s2 = -s1 - s0
+    sides = [s0, s1, s2]
+    ang = []
+    for k in range(3):
+        a, b = sides[(k + 2) % 3], sides[k]
+        ang.append(np.arccos(-(a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))))
+    corners = np.stack(ang, 1)  # T x 3, in the order of the triangle loop
+    np.subtract.at(ref, tri.reshape(-1), corners.reshape(-1))
+    err = np.abs(d - ref).max()
+    print(f"angle deficit err {err:.2e}; sum {d.sum():.15f} vs 2 pi {2 * np.pi:.15f}")
+    assert err <= 1e-13
+    assert d.sum() == pytest.approx(2.0 * np.pi, abs=1e-9)

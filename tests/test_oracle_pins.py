@@ -220,6 +220,56 @@ def test_zero_force_at_programmed_state(shell_c1):
 
 # --------------------------------------------------------- sums and dynamics
 
+def _reference_pick(lam3):
+    """calc_stresses.cpp:165-176 on ascending eigenvalues: drop the first of
+    least magnitude, keep the next two cyclically (indices)."""
+    m = int(np.argmin(np.abs(lam3)))
+    return (m + 1) % 3, (m + 2) % 3
+
+
+@pytest.mark.parametrize("sx,sy", [(1.03, 1.0), (0.97, 1.0), (1.02, 0.99), (0.98, 1.04)])
+def test_stress_closed_forms_stretched_flat_sheet(shell_c1, sx, sy):
+    """A flat sheet with abar = I, bbar = 0 stretched by (sx, sy), then
+    rotated rigidly by Q: the tensor (A/J) sum_alpha dW/dR_alpha (x) R_alpha
+    is A c_s (2 g0 sx/sy Qe1 Qe1^T + 2 g2 sy/sx Qe2 Qe2^T) with c_s =
+    h mu/(1 - nu), g0 = 2 nu (ex + ey) + 2 (1 - nu) ex, g2 the same with ex,
+    ey swapped (ex = sx^2 - 1), and the normal's eigenvalue 0, so the two
+    kept eigenpairs are these (in the reference's order), eigenvectors Qe1
+    and Qe2 with their largest component positive."""
+    sh = shell_c1
+    T, N = sh.T, sh.N
+    abar = np.concatenate([np.ones(T), np.zeros(T), np.ones(T)])
+    sysm = orc.System.from_shell(sh, abar=abar, bbar=np.zeros(3 * T))
+    x = sh.ref_pos.copy()
+    x[:N] *= sx
+    x[N:2 * N] *= sy
+    x[2 * N:] = 0.0
+    ang = 0.7
+    Q = np.array([[math.cos(ang), 0.0, math.sin(ang)], [0.0, 1.0, 0.0], [-math.sin(ang), 0.0, math.cos(ang)]])
+    Q = Q @ np.array([[math.cos(0.4), -math.sin(0.4), 0.0], [math.sin(0.4), math.cos(0.4), 0.0], [0.0, 0.0, 1.0]])
+    x = (Q @ x.reshape(3, N)).reshape(-1)
+    S = sysm.stresses(x)
+    nu = sh.settings.def_poisson_ratio
+    cs = sh.thickness * sh.shear_modulus / (1.0 - nu)
+    ex, ey = sx * sx - 1.0, sy * sy - 1.0
+    g0 = 2.0 * nu * (ex + ey) + 2.0 * (1.0 - nu) * ex
+    g2 = 2.0 * nu * (ex + ey) + 2.0 * (1.0 - nu) * ey
+    l1 = 2.0 * sh.areas * cs * g0 * sx / sy
+    l2 = 2.0 * sh.areas * cs * g2 * sy / sx
+    e1, e2 = Q[:, 0], Q[:, 1]
+    scale = np.maximum(np.abs(l1), np.abs(l2))
+    for t in range(0, T, 37):
+        vals = [(0.0, None), (l1[t], e1), (l2[t], e2)]
+        vals.sort(key=lambda p: p[0])
+        i1, i2 = _reference_pick(np.array([v[0] for v in vals]))
+        for r, k in enumerate((i1, i2)):
+            lam, vec = vals[k]
+            assert S[4 * r, t] == pytest.approx(lam, abs=1e-12 * scale[t])
+            v = S[4 * r + 1:4 * r + 4, t]
+            assert abs(abs(v @ vec) - 1.0) <= 1e-12
+            assert v[np.argmax(np.abs(v))] > 0.0
+
+
 def test_kahan_against_fsum():
     rng = np.random.default_rng(0)
     v = np.concatenate([[1.0], np.full(10000, 1e-16), rng.normal(size=1000) * 1e-3])
