@@ -334,6 +334,27 @@ typedef struct ms_relax_report {
                             applied to it; a halted shell's count stops) */
 } ms_relax_report;
 
+/* The protocol's log (main.cpp:303, 396-407; is_at_equilibrium.cpp:89-90):
+ * with a check log enabled, each shell records every equilibrium check
+ * (kind 0: step = the shell's own step count at the check, the dial factor
+ * used, the force and speed ratios with their first-argmax nodes, the
+ * verdict) and every end of a dialling phase (kind 1: the dial factor the
+ * shell starts waiting at) into a device ring of `capacity` records;
+ * ms_read_check_log returns the records since the previous read, oldest
+ * first, and how many were overwritten before they were read. */
+typedef struct ms_check_record {
+    int64_t step;
+    double dial_factor;
+    double max_force_ratio;
+    double max_speed_ratio;
+    int32_t max_force_node;
+    int32_t max_speed_node;
+    int32_t at_equilibrium;
+    int32_t kind;            /* 0 equilibrium check, 1 dialling phase ended */
+} ms_check_record;
+int ms_set_check_log(ms_ctx* ctx, int32_t capacity);  /* 0 disables */
+int ms_read_check_log(ms_ctx* ctx, int32_t shell, ms_check_record* out, int32_t max_records, int32_t* count,
+                      int64_t* dropped);
 /* Arm the protocol for one shell (the current state is the start). */
 int ms_relax_setup(ms_ctx* ctx, int32_t shell, const ms_relax_params* p);
 /* Step all shells until every armed shell has halted or max_steps elapsed,

@@ -91,6 +91,9 @@ struct ms_ctx {
     int64_t dev_bytes = 0;
     std::vector<void*> allocs;
 
+    // protocol event log (ms_set_check_log): per-shell rings, read positions
+    std::vector<long long> clog_read;
+
     // this device's address of the static scratch array of the allocation-free
     // entry point (null, with the reason in noalloc_err, if it could not be taken)
     double* noalloc_scratch = nullptr;
@@ -897,6 +900,7 @@ extern "C" void ms_destroy(ms_ctx* c) {
     if (c->scratch3) hipFree(c->scratch3);
     if (c->node_out) hipFree(c->node_out);
     if (c->node_flags) hipFree(c->node_flags);
+    if (c->b.clog) hipFree(c->b.clog);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1474,6 +1478,51 @@ extern "C" int ms_relax_run(ms_ctx* c, int64_t max_steps, int64_t poll_every, ms
         R.steps = S.own_steps;
     }
     c->err.clear();
+    return MS_OK;
+}
+
+static_assert(sizeof(CheckRec) == sizeof(ms_check_record), "check record layout");
+
+extern "C" int ms_set_check_log(ms_ctx* c, int32_t capacity) {
+    if (!c) return fail(nullptr, MS_ERR_ARG, "null context");
+    if (capacity < 0) return fail(c, MS_ERR_ARG, "ms_set_check_log: capacity < 0");
+    if (int r = pull_shells(c)) return r;
+    CheckRec* p = nullptr;
+    if (capacity > 0)
+        HIPC(c, hipMalloc((void**)&p, sizeof(CheckRec) * (size_t)capacity * c->num_shells));
+    destroy_graph(c);  // the step launches name the log
+    if (c->b.clog) hipFree(c->b.clog);
+    c->b.clog = p;
+    c->b.clog_cap = capacity;
+    c->clog_read.assign(c->num_shells, 0);
+    for (int s = 0; s < c->num_shells; ++s) c->clog_read[s] = c->hs[s].num_logged;
+    return MS_OK;
+}
+
+extern "C" int ms_read_check_log(ms_ctx* c, int32_t s, ms_check_record* out, int32_t max_records, int32_t* count,
+                                 int64_t* dropped) {
+    if (int r = shell_ok(c, s)) return r;
+    if (!count || (max_records > 0 && !out) || max_records < 0)
+        return fail(c, MS_ERR_ARG, "ms_read_check_log: bad arguments");
+    *count = 0;
+    if (dropped) *dropped = 0;
+    if (!c->b.clog) return fail(c, MS_ERR_ARG, "ms_read_check_log: no check log (ms_set_check_log)");
+    if (int r = pull_shells(c)) return r;
+    const long long total = c->hs[s].num_logged, cap = c->b.clog_cap;
+    long long from = c->clog_read[s];
+    if (total - from > cap) {
+        if (dropped) *dropped = total - from - cap;
+        from = total - cap;
+    }
+    const long long n = std::min<long long>(total - from, max_records);
+    for (long long i = 0; i < n;) {  // at most two runs around the ring's end
+        const long long slot = (from + i) % cap, run = std::min(n - i, cap - slot);
+        HIPC(c, hipMemcpy(out + i, c->b.clog + (size_t)s * cap + slot, sizeof(CheckRec) * run,
+                          hipMemcpyDeviceToHost));
+        i += run;
+    }
+    c->clog_read[s] = from + n;
+    *count = (int32_t)n;
     return MS_OK;
 }
 

@@ -71,6 +71,7 @@ struct ShellDev {
     // Per-shell uniform triangle data (bitwise equal over the shell's
     // triangles): the force kernels take them from here instead of streaming
     // the per-triangle arrays.  uni_b is set only for blend-protocol shells.
+    long long num_logged;  // protocol events written to the check log (ring index = count % capacity)
     int uni_hmu;           // thickness and shear modulus uniform: h_u, mu_u
     int uni_b;             // bbar_info uniform: b_u
     double h_u, mu_u;
@@ -89,6 +90,23 @@ __host__ __device__ inline double shell_dial_factor(const ShellDev& s) {
 // step launch may take a table of the running shells' chunks only; the index
 // keeps per-chunk partial results at their place in the full table.
 typedef int4 Chunk;
+
+// One event of the device-run protocol, as the reference's log reports it
+// (main.cpp:303, 396-407; is_at_equilibrium.cpp:89-90): kind 0 is an
+// equilibrium check at the shell's own step `step` (its input state) with
+// the dial factor used, the force and speed ratios and their first-argmax
+// nodes, and the verdict; kind 1 is the end of a dialling phase (the shell
+// starts waiting for equilibrium at dial factor `dial_factor` on step `step`).
+struct CheckRec {
+    long long step;
+    double dial_factor;
+    double max_force_ratio;
+    double max_speed_ratio;
+    int max_force_node;
+    int max_speed_node;
+    int at_equilibrium;
+    int kind;
+};
 
 // Global clock of a context: steps taken.
 struct ClockDev {

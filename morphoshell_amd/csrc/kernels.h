@@ -45,6 +45,8 @@ struct Buffers {
     double* part;             // per-chunk partial sums / maxima scratch
     int* part_i;
     long long* bad_dof;       // per shell, ops-path blow-up scan
+    CheckRec* clog;           // per-shell rings of clog_cap protocol events, or null
+    int clog_cap;
 };
 
 enum TriMode { TRI_FORCES = 0, TRI_ENERGY = 1, TRI_GEOMETRY = 2, TRI_DIAGNOSTICS = 3, TRI_STRESS = 4 };

@@ -199,6 +199,18 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
                 S->f_override = -1.0;
                 bool eq_event = false;
                 if (proto_check) {
+                    if (k.b.clog) {
+                        CheckRec& e = k.b.clog[(size_t)s * k.b.clog_cap + S->num_logged % k.b.clog_cap];
+                        e.step = S->own_steps;
+                        e.dial_factor = f_used;
+                        e.max_force_ratio = S->max_f;
+                        e.max_speed_ratio = S->max_v;
+                        e.max_force_node = S->f_node;
+                        e.max_speed_node = S->v_node;
+                        e.at_equilibrium = S->at_eq;
+                        e.kind = 0;
+                        S->num_logged += 1;
+                    }
                     S->since_check = 0;
                     if (S->at_eq) {
                         eq_event = true;
@@ -233,6 +245,16 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
                     S->dialling = 0;
                     S->since_check = 0;
                     S->eta = S->eta2;
+                    if (k.b.clog) {
+                        CheckRec& e = k.b.clog[(size_t)s * k.b.clog_cap + S->num_logged % k.b.clog_cap];
+                        e.step = S->own_steps + 1;
+                        e.dial_factor = shell_dial_factor(*S);
+                        e.max_force_ratio = e.max_speed_ratio = 0.0;
+                        e.max_force_node = e.max_speed_node = -1;
+                        e.at_equilibrium = 0;
+                        e.kind = 1;
+                        S->num_logged += 1;
+                    }
                 }
                 S->check_due = (S->status == 1 && S->since_check > S->equil_interval) ? 1 : 0;
                 if (S->status == 0) S->f_targets = shell_dial_factor(*S);

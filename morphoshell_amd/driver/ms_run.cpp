@@ -51,6 +51,14 @@ using namespace msrun;
 
 namespace {
 
+// get_real_time.cpp:31-40: day/month/year hour:minute.
+std::string real_time() {
+    const std::time_t t = std::time(nullptr);
+    char buf[32];
+    std::strftime(buf, sizeof buf, "%d/%m/%y %H:%M", std::localtime(&t));
+    return buf;
+}
+
 std::string now_string() {
     const std::time_t t = std::time(nullptr);
     char buf[64];
@@ -260,7 +268,7 @@ void write_job(const Run& r, const OutputJob& j) {
 // The device part of an output (state, curvatures, strain, energy
 // densities, Cauchy stress eigenpairs, angle deficits), copied to the host
 // and handed to the writer thread.
-void write_output(Run& r, int64_t stepcount, double dial_factor, int status) {
+void write_output(Run& r, int64_t stepcount, double dial_factor, int status, double time_plus_dt) {
     const int N = r.mesh.num_nodes, T = r.mesh.num_tris;
     OutputJob j{stepcount,
                 dial_factor,
@@ -287,6 +295,9 @@ void write_output(Run& r, int64_t stepcount, double dial_factor, int status) {
         j.stresses[4 * (size_t)T + t] /= mh;
     }
     r.writer->submit(std::move(j));
+    // main.cpp:437 prints the simulation time plus one timestep
+    r.log << "Wrote VTK output at " << real_time() << ", stepcount = " << stepcount
+          << ", simulation time = " << time_plus_dt << ", current dial factor = " << dial_factor << std::endl;
 }
 
 int run(int argc, char** argv) {
@@ -451,6 +462,44 @@ int run(int argc, char** argv) {
     rp.pause_on_eq = 1;
     check(ms_relax_setup(r.ctx, 0, &rp), "ms_relax_setup");
 
+    // every protocol check and phase end of the device run is logged as the
+    // reference's loop logs it (main.cpp:303, 396-407)
+    constexpr int32_t kLogCapacity = 16384;
+    check(ms_set_check_log(r.ctx, kLogCapacity), "ms_set_check_log");
+    const int64_t steps_per_call = (int64_t)kLogCapacity * std::max<int64_t>(1, rp.equil_interval + 1);
+    // simulation time of a step: the timestep changes when gradient descent ends
+    const double dt_gd = r.sc.timestep;
+    const double dt_newton = gd ? scales_for(r, r.st.damping_prefactor_1, false).timestep : r.sc.timestep;
+    const int64_t gd_steps = gd ? (int64_t)r.st.num_steps_gradient_descent + 1 : 0;
+    auto sim_time = [&](int64_t k) {
+        return k <= gd_steps ? (double)k * dt_gd : (double)gd_steps * dt_gd + (double)(k - gd_steps) * dt_newton;
+    };
+    auto time_plus_dt = [&](int64_t k) { return sim_time(k) + (k < gd_steps ? dt_gd : dt_newton); };
+    std::vector<ms_check_record> events(kLogCapacity);
+    auto drain_log = [&]() {
+        int32_t n = 0;
+        int64_t dropped = 0;
+        check(ms_read_check_log(r.ctx, 0, events.data(), kLogCapacity, &n, &dropped), "ms_read_check_log");
+        if (dropped) r.log << "(" << dropped << " protocol events not logged)" << std::endl;
+        for (int32_t i = 0; i < n; ++i) {
+            const ms_check_record& e = events[i];
+            if (e.kind == 1) {
+                r.log << "Reached dial_factor = " << e.dial_factor << ", now waiting for equilibrium." << std::endl;
+                continue;
+            }
+            r.log << "Checking for equilibrium at " << real_time() << ", stepcount = " << e.step
+                  << ", simulation time = " << sim_time(e.step) << ", dial_factor = " << e.dial_factor << std::endl;
+            r.log << "Max ratio of non-damping force to characteristic force = " << e.max_force_ratio << " (node "
+                  << e.max_force_node << ")" << "\n";
+            r.log << "Max ratio of node speed to characteristic speed = " << e.max_speed_ratio << " (node "
+                  << e.max_speed_node << ")" << "\n";
+            if (e.at_equilibrium)
+                r.log << "Equilibrium reached. Writing VTK output at " << real_time() << ", stepcount = " << e.step
+                      << ", simulation time = " << sim_time(e.step) << ", dial_factor = " << e.dial_factor
+                      << std::endl;
+        }
+    };
+
     r.writer = std::make_unique<OutputWriter>([&r](const OutputJob& j) { write_job(r, j); });
     r.log.precision(8, true);
     r.log << "Beginning dynamics loop." << std::endl;
@@ -459,30 +508,28 @@ int run(int argc, char** argv) {
     const int64_t gd_until = gd ? (int64_t)r.st.num_steps_gradient_descent + 1 : -1;
     const int64_t limit = r.st.max_steps > 0 ? r.st.max_steps : INT64_MAX;
     int64_t steps = 0;
-    if (every > 0) write_output(r, 0, dial_start, 0);
+    if (every > 0) write_output(r, 0, dial_start, 0, time_plus_dt(0));
     ms_relax_report rep{};
     while (true) {
         int64_t target = limit;
         if (every > 0) target = std::min(target, (steps / every + 1) * every);
         if (gd_until > steps) target = std::min(target, gd_until);
-        const int64_t n = target - steps;
+        const int64_t n = std::min(target - steps, steps_per_call);  // the log ring holds one call's events
         if (n <= 0) break;
         check(ms_relax_run(r.ctx, n, std::min<int64_t>(n, 50000), &rep), "ms_relax_run");
+        drain_log();
         steps = rep.steps;
         if (rep.failed) {
             // main.cpp:440-470: output of the failing step's input, then the error
-            write_output(r, steps, rep.dial_factor, rep.status);
+            write_output(r, steps, rep.dial_factor, rep.status, time_plus_dt(steps));
             r.writer->finish();
             r.log << "At stepcount = " << steps << ", dial_factor = " << rep.dial_factor
                   << " there was a problem: implausibly_high_force" << std::endl;
             return -1;
         }
         if (rep.halted) {
-            // an equilibrium: the checked state is the current one
-            r.log << "Equilibrium reached at stepcount = " << rep.eq_step << ", dial_factor = " << rep.dial_factor
-                  << ", max force ratio " << rep.max_force_ratio << ", max speed ratio " << rep.max_speed_ratio
-                  << std::endl;
-            write_output(r, rep.eq_step, rep.dial_factor, 2);
+            // an equilibrium (logged with its check): the checked state is the current one
+            write_output(r, rep.eq_step, rep.dial_factor, 2, time_plus_dt(rep.eq_step));
             if (rep.status == 2) break;  // dial factor 1: the run is complete
             check(ms_resume_shell(r.ctx, 0), "ms_resume_shell");
             steps = rep.eq_step + 1;  // resuming applies the checked step's update
@@ -498,7 +545,8 @@ int run(int argc, char** argv) {
             check(ms_set_shell_params(r.ctx, 0, &p2), "ms_set_shell_params");
             every = s2.steps_between_outputs;
         }
-        if (every > 0 && steps % every == 0) write_output(r, steps, rep.dial_factor, rep.status);
+        if (every > 0 && steps % every == 0)
+            write_output(r, steps, rep.dial_factor, rep.status, time_plus_dt(steps));
         if (steps >= limit) {
             r.log << "Stopped after max_steps = " << limit << " steps." << std::endl;
             break;
@@ -508,8 +556,9 @@ int run(int argc, char** argv) {
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     double E[3] = {0, 0, 0};
     check(ms_calc_energies(r.ctx, 0, E), "calc_energies");
-    r.log << "Simulation completed at stepcount = " << rep.steps << " (" << secs << " s of dynamics); energies "
-          << E[0] << " " << E[1] << " " << E[2] << std::endl;
+    r.log << "Simulation completed at time = " << sim_time(rep.steps) << ", stepcount = " << rep.steps << std::endl;
+    r.log << "(" << secs << " s of dynamics on the device; energies " << E[0] << " " << E[1] << " " << E[2] << ")"
+          << std::endl;
     return 0;
 }
 

@@ -28,7 +28,7 @@ EXPORTED = [
     "ms_calc_geometry", "ms_calc_geometry_noalloc", "ms_calc_stresses", "ms_calc_angle_deficits", "ms_calc_energy_densities", "ms_calc_diagnostics", "ms_get_forces", "ms_set_forces", "ms_step",
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
     "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_calibration_read",
-    "ms_device_info", "ms_relax_setup", "ms_relax_run", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
+    "ms_device_info", "ms_relax_setup", "ms_relax_run", "ms_set_check_log", "ms_read_check_log", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
     "ms_nn_set_params", "ms_nn_set_graph", "ms_nn_forward", "ms_nn_loss_backward", "ms_nn_adam_step",
     "ms_nn_gemm_test",
 ]
@@ -135,6 +135,17 @@ class RelaxReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class CheckRecord(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("dial_factor", C.c_double), ("max_force_ratio", C.c_double),
+        ("max_speed_ratio", C.c_double), ("max_force_node", C.c_int32), ("max_speed_node", C.c_int32),
+        ("at_equilibrium", C.c_int32), ("kind", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -198,6 +209,9 @@ def lib() -> C.CDLL:
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "ms_relax_setup": (C.c_int, [P, C.c_int32, C.POINTER(RelaxParams)]),
         "ms_relax_run": (C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(RelaxReport)]),
+        "ms_set_check_log": (C.c_int, [P, C.c_int32]),
+        "ms_read_check_log": (C.c_int, [P, C.c_int32, C.POINTER(CheckRecord), C.c_int32,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
         "ms_nn_create": (C.c_int, [C.POINTER(NNArch), C.POINTER(P)]),
         "ms_nn_destroy": (None, [P]),
         "ms_nn_last_error": (C.c_char_p, [P]),

@@ -528,6 +528,19 @@ class Context:
         self._ck(self.L.ms_relax_run(self.h, int(max_steps), int(poll_every), reps))
         return [r.as_dict() for r in reps]
 
+    def set_check_log(self, capacity: int) -> None:
+        """Record every protocol equilibrium check and dialling-phase end of
+        each shell in a device ring of `capacity` records (0 turns it off)."""
+        self._ck(self.L.ms_set_check_log(self.h, int(capacity)))
+
+    def read_check_log(self, shell: int, max_records: int = 1 << 20) -> tuple[list[dict], int]:
+        """Protocol events of the shell since the previous read, oldest first,
+        and the number overwritten before they were read."""
+        buf = (nat.CheckRecord * max_records)()
+        n, dropped = C.c_int32(0), C.c_int64(0)
+        self._ck(self.L.ms_read_check_log(self.h, shell, buf, max_records, C.byref(n), C.byref(dropped)))
+        return [buf[i].as_dict() for i in range(n.value)], int(dropped.value)
+
     @property
     def steps_taken(self) -> int:
         return int(self.L.ms_steps_taken(self.h))

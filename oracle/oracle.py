@@ -249,9 +249,13 @@ def run_protocol(sysm: System, dial, x, v, K, spacing, start, interval, eta1, et
     whose dial factor is 1 (the reference's own loop never stops there,
     main.cpp:493).  ``dial(f)`` returns (abar, bbar) 3T SoA at dial factor f.
 
-    Returns dict(steps, eq_steps, status, dial_factor) and updates x, v in place
-    (x, v end as the checked state of the final equilibrium, or after
-    max_steps).
+    Returns dict(steps, eq_steps, status, dial_factor, events) and updates x, v
+    in place (x, v end as the checked state of the final equilibrium, or
+    after max_steps).  events lists what the reference's loop logs, in order:
+    ("phase_end", step, dial factor) when a dialling phase ends
+    (main.cpp:293-304) and ("check", step, dial factor, force ratio, its node,
+    speed ratio, its node, verdict) at every equilibrium check
+    (main.cpp:396-407, is_at_equilibrium.cpp:89-90).
     """
     import math
 
@@ -263,6 +267,7 @@ def run_protocol(sysm: System, dial, x, v, K, spacing, start, interval, eta1, et
     f = start
     since_check = -123456
     eq_steps = []
+    events = []
     lib().orc_set_eta(sysm.h, eta1)
     while step < max_steps:
         if f >= 1.0:
@@ -271,22 +276,25 @@ def run_protocol(sysm: System, dial, x, v, K, spacing, start, interval, eta1, et
             status = WAIT
             since_check = 0
             lib().orc_set_eta(sysm.h, eta2)
+            events.append(("phase_end", step, f))
         if status == DIAL or step == 0:
             ab, bb = dial(f)
             sysm.set_targets(ab, bb)
         F = sysm.full_forces(x, v)
         if status == WAIT and since_check > interval:
-            fr, _, vr, _ = sysm.equilibrium(F, v)
+            fr, fi, vr, vi = sysm.equilibrium(F, v)
+            events.append(("check", step, f, fr, fi, vr, vi, fr < tol_f and vr < tol_v))
             if fr < tol_f and vr < tol_v:
                 status = EQ
                 last_eq = f
                 eq_steps.append(step)
             since_check = 0
         if status == EQ and f >= 1.0:
-            return {"steps": step, "eq_steps": eq_steps, "status": 2, "dial_factor": f}
+            return {"steps": step, "eq_steps": eq_steps, "status": 2, "dial_factor": f, "events": events}
         d, _ = sysm.advance(x, v, F)
         if d >= 0:
-            return {"steps": step, "eq_steps": eq_steps, "status": -1, "dial_factor": f, "bad_dof": d}
+            return {"steps": step, "eq_steps": eq_steps, "status": -1, "dial_factor": f, "bad_dof": d,
+                    "events": events}
         step += 1
         if status == DIAL:
             phase += 1
@@ -297,4 +305,4 @@ def run_protocol(sysm: System, dial, x, v, K, spacing, start, interval, eta1, et
             status = DIAL
             phase = 0
             lib().orc_set_eta(sysm.h, eta1)
-    return {"steps": step, "eq_steps": eq_steps, "status": status, "dial_factor": f}
+    return {"steps": step, "eq_steps": eq_steps, "status": status, "dial_factor": f, "events": events}

@@ -188,6 +188,20 @@ def test_driver_run_matches_context_relax(tmp_path, spacing):
         assert np.allclose(blocks[ev][:, 0], lam, rtol=1e-13, atol=1e-13 * np.abs(lam).max())
         assert np.allclose(blocks[vec], S[4 * r + 1:4 * r + 4].T, rtol=1e-13, atol=1e-13)
     assert np.allclose(deficit, d_ctx, rtol=1e-13, atol=1e-13)
+    # log.txt carries the reference's per-check lines (main.cpp:396-407,
+    # is_at_equilibrium.cpp:89-90) and phase ends (main.cpp:303)
+    log = (out / "log.txt").read_text().split("\n")
+    checks = [i for i, ln in enumerate(log) if ln.startswith("Checking for equilibrium at ")]
+    assert len(checks) >= n_eq
+    for i in checks:
+        assert log[i + 1].startswith("Max ratio of non-damping force to characteristic force = ")
+        assert log[i + 2].startswith("Max ratio of node speed to characteristic speed = ")
+    reached = [ln for ln in log if ln.startswith("Equilibrium reached. Writing VTK output at ")]
+    assert len(reached) == n_eq and f"stepcount = {last_step}," in reached[-1]
+    assert sum(ln.endswith(", now waiting for equilibrium.") for ln in log) == n_eq
+    assert sum(ln.startswith("Wrote VTK output at ") for ln in log) == n_eq
+    assert any(ln.startswith(f"Simulation completed at time = ") and ln.endswith(f"stepcount = {last_step}")
+               for ln in log)
 
 
 @pytest.mark.gpu

@@ -4,8 +4,8 @@ the oracle's restatement of the same loop, through the C-ABI.
 Shell: 711-triangle disc, GRF metric (amplitude 0.05), bbar = 0.3 I, h = 0.05,
 damping prefactors 20, two dialling phases of 300 steps (spacing 0.5),
 equilibrium checks every 25 steps with both tolerances 1e-2.  The oracle
-reaches equilibria at steps 4304 and 6113; the device must reach them at the
-same steps and end in the same state (positions within 1e-9 after ~6000
+reaches equilibria at steps 3316 and 5125; the device must reach them at the
+same steps and end in the same state (positions within 1e-9 after ~5000
 FP64 steps).
 """
 import numpy as np
@@ -63,6 +63,46 @@ def test_device_protocol_matches_oracle():
     assert rep["dial_factor"] == 1.0
     assert np.abs(xg - xo).max() <= 1e-9
     assert np.abs(vg - vo).max() <= 1e-9
+
+
+def test_check_log_matches_the_oracle_loop():
+    """The device's protocol log (ms_set_check_log / ms_read_check_log)
+    against the events of the oracle's restatement of the loop: every
+    dialling-phase end and every equilibrium check, in order, at the same
+    steps with the same dial factors, first-argmax nodes and verdicts, and
+    the force and speed ratios to 1e-9 relative (the FP64 states agree to
+    ~1e-11).  Read in pieces over the run, with a ring too small to hold it
+    all at once."""
+    from morphoshell_amd.shell import Context
+
+    sh, x = _setup()
+    r, _, _ = _oracle(sh, x)
+    ev_o = r["events"]
+    checks = sum(e[0] == "check" for e in ev_o)
+    assert checks > 50 and sum(e[0] == "phase_end" for e in ev_o) == 2
+    got = []
+    with Context(sh) as ctx:
+        ctx.set_tuning(node_block=128, fused=1)
+        ctx.set_check_log(64)
+        ctx.set_state(0, x)
+        _arm(ctx, 0)
+        while True:
+            rep = ctx.relax_run(1000, poll_every=500)[0]  # at most 1000 / (INTERVAL + 1) < 64 events
+            evs, dropped = ctx.read_check_log(0)
+            assert dropped == 0
+            got += evs
+            if rep["halted"]:
+                break
+    assert len(got) == len(ev_o)
+    for g, o in zip(got, ev_o):
+        if o[0] == "phase_end":
+            assert g["kind"] == 1 and g["step"] == o[1] and g["dial_factor"] == o[2], (g, o)
+        else:
+            _, step, f, fr, fi, vr, vi, ok = o
+            assert g["kind"] == 0 and g["step"] == step and g["dial_factor"] == f, (g, o)
+            assert (g["max_force_node"], g["max_speed_node"], bool(g["at_equilibrium"])) == (fi, vi, ok), (g, o)
+            assert g["max_force_ratio"] == pytest.approx(fr, rel=1e-9)
+            assert g["max_speed_ratio"] == pytest.approx(vr, rel=1e-9)
 
 
 def test_pause_and_resume_equals_uninterrupted():
