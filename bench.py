@@ -161,58 +161,48 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(sh, x, v, budget_s: float) -> dict:
-    """The FP64 oracle's step throughput on the host: the median of up to 5
-    runs of 100 steps with every OpenMP thread (fewer runs if the budget is
-    short), and a shorter run on one thread."""
-    from oracle import oracle as orc
+def cpu_baseline(spacing: float, runs: int = 5, steps: int = 100) -> dict:
+    """The FP64 oracle's step throughput on this host, as BASELINE.md plans
+    it: oracle.c built here with -march=native (the reference's Makefile
+    flag), run in a child process with its OpenMP threads pinned
+    (OMP_PLACES=cores, OMP_PROC_BIND=spread, the reference's recommended
+    launch) on every CPU this process may use; the median of `runs` runs of
+    `steps` steps after warm-up, and a shorter one-thread run."""
+    import subprocess
+    import tempfile
 
-    L = orc.lib()
-    threads = L.orc_num_threads()
-    sysm = orc.System.from_shell(sh)
-    xx, vv = x.copy(), v.copy()
-
-    def steps(n):
-        if sysm.run_steps(xx, vv, n) != -1:
-            raise RuntimeError("CPU baseline: the oracle's step hit the blow-up test; the timing would be void")
-
-    t0 = time.perf_counter()
-    steps(2)  # warm-up
-    t1 = (time.perf_counter() - t0) / 2
-    n = 100
-    runs = int(max(1, min(5, 0.8 * budget_s / (n * t1))))
-    rates = []
-    for _ in range(runs):
-        t0 = time.perf_counter()
-        steps(n)
-        rates.append(sh.T * n / (time.perf_counter() - t0))
-    L.orc_set_num_threads(1)
     try:
-        t0 = time.perf_counter()
-        steps(1)
-        t_one = time.perf_counter() - t0
-        n1 = int(max(2, min(20, 0.2 * budget_s / t_one)))
-        t0 = time.perf_counter()
-        steps(n1)
-        one = sh.T * n1 / (time.perf_counter() - t0)
-    finally:
-        L.orc_set_num_threads(threads)
-    try:
-        logical = len(os.sched_getaffinity(0))
+        cpus = len(os.sched_getaffinity(0))
     except AttributeError:
-        logical = os.cpu_count()
+        cpus = os.cpu_count() or 1
+    with tempfile.TemporaryDirectory() as tmp:
+        lib = os.path.join(tmp, "liboracle_native.so")
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), f"OUT={lib}", "ARCH=native"],
+                       check=True, capture_output=True)
+        # KMP_AFFINITY=disabled: the set-up of the shell runs the product
+        # library's OpenMP (LLVM libomp), which would otherwise bind the main
+        # thread to the first place; the oracle's runtime (GNU libgomp, which
+        # ignores KMP_*) would then see one CPU and put its whole team there
+        env = dict(os.environ, ORACLE_LIB=lib, OMP_PLACES="cores", OMP_PROC_BIND="spread",
+                   OMP_NUM_THREADS=str(cpus), KMP_AFFINITY="disabled")
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_throughput.py"), "--spacing",
+                            str(spacing), "--runs", str(runs), "--steps", str(steps)],
+                           env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("CPU baseline failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
+        d = json.loads(r.stdout.strip().splitlines()[-1])
     return {
-        "value": float(np.median(rates)),
+        "value": d["median"],
         "unit": "triangle force-evals/s",
-        "cores": threads,
+        "cores": d["threads"],
         "kind": "port",
-        "sample": f"median of {runs} runs of {n} FP64 steps of the same {sh.T}-triangle shell (q=M x, dW/dq, "
-                  f"F=-M^T g, damping, clamp, update) with {threads} OpenMP threads (not pinned: binding "
-                  f"would also pin the process's own threads and the meshing workers it forks)",
-        "runs": [float(r) for r in rates],
-        "one_thread": {"value": one, "steps": n1},
+        "sample": f"median of {runs} runs of {steps} FP64 steps of the same {d['T']}-triangle shell (q=M x, dW/dq, "
+                  f"F=-M^T g, damping, clamp, update) by the oracle built with -march=native, "
+                  f"{d['threads']} OpenMP threads pinned (OMP_PLACES=cores, OMP_PROC_BIND=spread)",
+        "runs": d["runs"],
+        "one_thread": {"value": d["one_thread"], "steps": d["one_thread_steps"]},
         "cpu_model": cpu_model(),
-        "logical_cpus_available": logical,
+        "logical_cpus_available": cpus,
     }
 
 
@@ -289,10 +279,15 @@ def bench_single(args) -> dict:
         log(f"copy kernel 1 GiB: {copy_gbs:.0f} GB/s; read-only stream 1 GiB: {read_gbs:.0f} GB/s")
     log(f"device {device}")
 
+    tte = None
+    if not args.no_relax:
+        log("time to equilibrium of the small defect disc")
+        tte = c1_relaxation(args.relax_steps, warm=True, cpu_steps=2000 if not args.no_cpu else 0)
+
     cpu, parity = None, {}
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(sh, x, v, args.cpu_budget)
+        cpu = cpu_baseline(args.spacing)
         log(f"cpu {cpu['value'] / 1e6:.2f} M tri-evals/s on {cpu['cores']} threads")
         if not args.no_parity:
             parity = baseline_parity(sh, x, v)
@@ -370,6 +365,7 @@ def bench_single(args) -> dict:
         "forms": {f"{p}_{f}": summary(p, f) for (p, f) in res},
         "parity": parity,
         "cpu_baseline": cpu,
+        "time_to_equilibrium": tte,
         "setup_s": t_setup,
         "device": device,
     }
@@ -426,7 +422,7 @@ def bench_batch(args) -> dict:
             ctx.relax_setup(i, steps_per_dial_phase=1000, dial_spacing=1.0, dial_start=0.0, equil_interval=200,
                             tol_f=1e-6, tol_v=1e-5)
         t1 = time.perf_counter()
-        reps = ctx.relax_run(args.relax_steps, poll_every=2000) if args.relax_steps > 0 else \
+        reps = ctx.relax_run(args.batch_steps, poll_every=2000) if args.batch_steps > 0 else \
             [{"status": 0, "failed": 0, "max_force_ratio": float("nan")} for _ in shells]
         t_relax = time.perf_counter() - t1
     done = sum(1 for r in reps if r["status"] == 2)
@@ -462,7 +458,7 @@ def bench_batch(args) -> dict:
         "dominant_measured_traffic": batch_traffic(tuned, kt[0]),
         "step_alg_GBs": step_bytes(T, N, True, uni) / (secs / args.steps) / 1e9,
         "tri_record_bytes": tri_bytes(True, uni),
-        "relax": {"steps": args.relax_steps, "seconds": t_relax, "shells_at_equilibrium": done,
+        "relax": {"steps": args.batch_steps, "seconds": t_relax, "shells_at_equilibrium": done,
                   "shells_failed": failed, "shells_per_s": done / t_relax if t_relax > 0 else 0.0,
                   "tolerances": {"force": 1e-6, "speed": 1e-5},
                   "shells_with_last_force_ratio_at_most": at_tol,
@@ -815,7 +811,33 @@ def bench_c5(args) -> dict:
     return out
 
 
-def bench_relax(args) -> dict:
+# Launch shape of small shells: a triangle and a node launch of 64-wide
+# blocks on persistent grids (what the driver and the autotuner pick below
+# ~100k triangles; such shells are launch-bound).
+SMALL_SHAPE = dict(tri_block=64, node_block=64, tri_grid_per_cu=8, node_grid_per_cu=8, graph_steps=16, fused=0)
+
+
+def c1_relaxation(max_steps: int, shape64: dict | None = None, shape_mixed: dict | None = None,
+                  warm: bool = True, cpu_steps: int = 2000) -> dict:
+    """Time to equilibrium of the small defect disc (4510 triangles, LCE
+    lambda 1 -> 0.9 over 20k steps, h = 0.01) under the device-run protocol
+    (main.cpp:269-507), to max |F|/F_char < 1e-6 and max |v|/c_char < 1e-5:
+      * fp64: plain FP64 from the flat start with z-noise;
+      * mixed_then_fp64: mixed-precision forces until the residual reaches
+        1e-4 F_char (or stops falling: FP32 force noise floors it), then FP64
+        to the tolerances from that state;
+      * warm_mixed_then_fp64: the learned warm start (shipped parameters,
+        with the energy-test fallback to the flat start), dialled from its
+        own geometry, then as mixed_then_fp64.
+    Each leg reports steps, seconds (prediction included), its end state's
+    residual and force error against the FP64 oracle, and its elastic
+    energy relative to the FP64 leg's.  The oracle runs the same protocol
+    on the host for cpu_steps steps; its time is extrapolated to the FP64
+    leg's step count."""
+    import copy
+    import dataclasses
+
+    from morphoshell_amd import warmstart as ws
     from morphoshell_amd import workloads
     from morphoshell_amd.shell import Context
     from oracle import oracle as orc
@@ -826,89 +848,148 @@ def bench_relax(args) -> dict:
     K = st.timesteps_per_dial_phase
     interval = max(1, int(round(st.time_between_equil_checks_prefactor * K)))
     tol_f, tol_v = st.force_to_char_force_ratio_equil_threshold, st.speed_to_char_speed_ratio_equil_threshold
+    shape64 = shape64 or SMALL_SHAPE
+    shape_mixed = shape_mixed or SMALL_SHAPE
+    mixed_tol = 1e-4
+    sysm = orc.System.from_shell(sh)
+    # While waiting for equilibrium the reference keeps the targets of the last
+    # dialled step, factor 1 - 1/K (do_dialling runs only while dialling,
+    # main.cpp:334-353); every leg ends at those targets, and the oracle
+    # checks the end states there (LCE protocol, do_dialling.cpp:104-124).
+    f_end = 1.0 - 1.0 / K
+    dial_end = orc.dial(sh.T, 3, False, f_end, sh.mesh.abar_info, None)
     log(f"small defect disc T={sh.T}, dt={sc['timestep']:.3e}, K={K}, check every {interval}")
 
-    # mixed-mode forces carry an FP32 error of ~1e-6..1e-5 F_char near
-    # equilibrium, so the mixed phase stops at a looser residual and FP64
-    # finishes to the real tolerances
-    mixed_tol = 1e-4
+    fchar = sc["char_force_scale"]
 
-    def arm(ctx, tf=tol_f, tv=tol_v):
+    def finish(ctx, t0, extra, targets):
+        """End state of a leg (at its final equilibrium): energy, and the
+        oracle's residual and force error there (in units of F_char: at an
+        equilibrium max |F| itself is ~1e-6 F_char) with the leg's targets."""
+        secs = time.perf_counter() - t0
+        x, v = ctx.get_state(0)
+        E = ctx.calc_energies(0)
+        d = {"seconds": secs, "energy": float(E[0] + E[1])}
+        d.update(extra)
+        if extra["status"] == 2:
+            ctx.set_precision("fp64")
+            ctx.calc_deformation_forces()
+            Fd = ctx.get_forces(0)
+            sysm.set_targets(*targets)
+            Fo = sysm.deformation_forces(x)
+            fr, _, vr, _ = sysm.equilibrium(sysm.full_forces(x, v), v)
+            d.update(oracle_force_ratio_at_end=fr, oracle_speed_ratio_at_end=vr,
+                     force_err_vs_oracle_over_fchar=float(np.abs(Fd - Fo).max() / fchar))
+        return d, x
+
+    legs = {}
+    # plain FP64
+    with Context(sh, precision="fp64") as ctx:
+        ctx.set_tuning(**shape64)
         ctx.set_state(0, x0, v0)
         ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=interval,
-                        tol_f=tf, tol_v=tv)
-
-    out = {"metric": "time-to-equilibrium", "unit": "s", "higher_is_better": False,
-           "config": {"workload": f"small defect disc, {sh.T} triangles, LCE lambda 1->0.9 over {K} steps, "
-                                  f"h=0.01, force tol {tol_f}, speed tol {tol_v}"}}
-    from morphoshell_amd import autotune
-
-    space = dict(autotune.DEFAULT_SPACE, node_block=(64, 128), fused_node_block=(64, 128))
-    with Context(sh, precision="fp64") as ctx:
-        ctx.set_state(0, x0, v0)
-        tuned64 = autotune.tune(ctx, steps=400, space=space, use_cache=False)
-        log(f"fp64 tuning: {tuned64}")
-        arm(ctx)
-        t0 = time.perf_counter()
-        r64 = ctx.relax_run(args.relax_steps, poll_every=20000)[0]
-        t64 = time.perf_counter() - t0
-        x64, v64 = ctx.get_state(0)
-        E64 = ctx.calc_energies(0)
-    log(f"fp64: {r64} in {t64:.2f}s")
-    with Context(sh, precision="mixed") as ctx:
-        ctx.set_state(0, x0, v0)
-        tunedm = autotune.tune(ctx, steps=400, space=space, use_cache=False)
-        arm(ctx, mixed_tol, mixed_tol)
-        t0 = time.perf_counter()
-        # the mixed phase ends at its tolerance or when its residual stops
-        # falling (FP32 force noise floors it), whichever comes first
-        chunk, best = 100_000, float("inf")
-        while True:
-            rm = ctx.relax_run(chunk, poll_every=20000)[0]
-            if rm["status"] == 2 or rm["failed"] or rm["steps"] >= args.relax_steps:
-                break
-            if rm["status"] == 1 and rm["max_force_ratio"] > 0.9 * best:
-                break
-            best = min(best, rm["max_force_ratio"]) if rm["status"] == 1 else best
-        steps_mixed = rm["steps"]
-        ctx.set_precision("fp64")
-        # continue from the mixed end state, already fully dialled
-        ctx.relax_setup(0, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=interval,
                         tol_f=tol_f, tol_v=tol_v)
-        rf = ctx.relax_run(args.relax_steps, poll_every=5000)[0]
-        tm = time.perf_counter() - t0
-        xm, _ = ctx.get_state(0)
-        Em = ctx.calc_energies(0)
-        steps_total = rm["steps"] + rf["steps"]
-    log(f"mixed: {rm}, then fp64 {rf}, {tm:.2f}s")
-    # the CPU oracle runs the same protocol for a bounded time; the rest is
-    # extrapolated at its measured per-step rate
-    sysm = orc.System.from_shell(sh)
-    xx, vv = x0.copy(), v0.copy()
-    budget = args.cpu_budget
-    steps_cpu = 2000
-    t0 = time.perf_counter()
-    orc.run_protocol(sysm, lambda f: orc.dial(sh.T, 3, False, f, sh.mesh.abar_info, None), xx, vv, K, 1.0, 0.0,
-                     interval, sc["damping_factor"], sc["damping_factor"], tol_f, tol_v, steps_cpu)
-    cpu_per_step = (time.perf_counter() - t0) / steps_cpu
-    n64 = (r64["eq_step"] + 1) if r64["status"] == 2 else args.relax_steps
-    out.update({
-        "fp64": {"seconds": t64, "steps": n64, "status": r64["status"], "max_force_ratio": r64["max_force_ratio"],
-                 "energy": (E64[0] + E64[1])},
-        "mixed_then_fp64": {"seconds": tm, "steps_mixed": steps_mixed, "steps_total": steps_total,
-                            "status": rf["status"], "max_force_ratio": rf["max_force_ratio"],
-                            "energy": (Em[0] + Em[1]),
-                            "energy_rel_diff_vs_fp64": abs((Em[0] + Em[1]) - (E64[0] + E64[1])) /
-                            abs(E64[0] + E64[1]),
-                            "max_position_diff_vs_fp64": float(np.abs(xm - x64).max())},
-        "speedup_mixed_vs_fp64": t64 / tm if tm > 0 else None,
-        "tuning": {"fp64": tuned64, "mixed": tunedm},
-        "cpu_baseline": {"seconds_extrapolated": cpu_per_step * n64, "per_step_s": cpu_per_step,
-                         "cores": orc.lib().orc_num_threads(), "kind": "port",
-                         "sample": f"{steps_cpu} steps of the same protocol, extrapolated to {n64} steps"},
-    })
-    del budget
-    return out
+        t0 = time.perf_counter()
+        r64 = ctx.relax_run(max_steps, poll_every=50000)[0]
+        legs["fp64"], x64 = finish(ctx, t0, {
+            "steps": (r64["eq_step"] + 1) if r64["status"] == 2 else r64["steps"], "status": r64["status"],
+            "max_force_ratio": r64["max_force_ratio"]}, dial_end)
+    log(f"fp64: {legs['fp64']}")
+
+    def mixed_leg(shell, x, v, dial_from_ansatz, t_extra=0.0):
+        with Context(shell, precision="mixed") as ctx:
+            ctx.set_tuning(**shape_mixed)
+            ctx.set_state(0, x, v)
+            targets = dial_end
+            if dial_from_ansatz:
+                # the start's a, b are the targets at dial factor 0 (do_dialling.cpp:194-228)
+                a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
+                ctx.set_zero_dial_geometry(0, a.reshape(-1), b.reshape(-1))
+                targets = orc.dial(sh.T, 3, True, f_end, sh.mesh.abar_info, None, a.reshape(-1), b.reshape(-1))
+            ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=interval,
+                            tol_f=mixed_tol, tol_v=mixed_tol)
+            t0 = time.perf_counter() - t_extra
+            chunk, best = 100_000, float("inf")
+            while True:
+                rm = ctx.relax_run(chunk, poll_every=50000)[0]
+                if rm["status"] == 2 or rm["failed"] or rm["steps"] >= max_steps:
+                    break
+                if rm["status"] == 1 and rm["max_force_ratio"] > 0.9 * best:
+                    break
+                best = min(best, rm["max_force_ratio"]) if rm["status"] == 1 else best
+            # FP64 from the mixed end state: one step dials the targets to
+            # f_end, then it waits for equilibrium at dial factor 1
+            ctx.set_precision("fp64")
+            ctx.relax_setup(0, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=f_end, equil_interval=interval,
+                            tol_f=tol_f, tol_v=tol_v)
+            rf = ctx.relax_run(max_steps, poll_every=50000)[0]
+            return finish(ctx, t0, {
+                "steps_mixed": rm["steps"], "steps_fp64_finish": rf["steps"],
+                "steps": rm["steps"] + ((rf["eq_step"] + 1) if rf["status"] == 2 else rf["steps"]),
+                "status": rf["status"], "max_force_ratio": rf["max_force_ratio"]}, targets)
+
+    legs["mixed_then_fp64"], xm = mixed_leg(sh, x0, v0, False)
+    legs["mixed_then_fp64"]["max_position_diff_vs_fp64"] = float(np.abs(xm - x64).max())
+    log(f"mixed then fp64: {legs['mixed_then_fp64']}")
+    if warm:
+        t0 = time.perf_counter()
+        net = ws.WarmStartNet.load(ws.PRETRAINED_DEFECT)
+        plans, used = ws.warm_start_plans(net, [sh], dial_steps=K, start="ansatz")
+        net.close()
+        t_pred = time.perf_counter() - t0
+        if plans[0] is not None:
+            shw = copy.copy(sh)
+            shw.settings = dataclasses.replace(sh.settings, dialling_from_ansatz_rather_than_ref_state=True)
+            legs["warm_mixed_then_fp64"], xw = mixed_leg(shw, plans[0][0], None, True, t_extra=t_pred)
+        else:
+            legs["warm_mixed_then_fp64"], xw = mixed_leg(sh, x0, v0, False, t_extra=t_pred)
+        legs["warm_mixed_then_fp64"]["max_position_diff_vs_fp64"] = float(np.abs(xw - x64).max())
+        legs["warm_mixed_then_fp64"].update(prediction_used=plans[0] is not None, alpha=used[0],
+                                            predict_s=t_pred)
+        log(f"warm start + mixed then fp64: {legs['warm_mixed_then_fp64']}")
+    e64 = legs["fp64"]["energy"]
+    for k in legs:
+        legs[k]["energy_rel_diff_vs_fp64"] = abs(legs[k]["energy"] - e64) / abs(e64)
+        legs[k]["speedup_vs_fp64"] = legs["fp64"]["seconds"] / legs[k]["seconds"]
+    n64 = legs["fp64"]["steps"]
+    cpu = None
+    if cpu_steps > 0:  # the oracle on the host: the same protocol for cpu_steps steps
+        xx, vv = x0.copy(), v0.copy()
+        t0 = time.perf_counter()
+        orc.run_protocol(sysm, lambda f: orc.dial(sh.T, 3, False, f, sh.mesh.abar_info, None), xx, vv, K, 1.0,
+                         0.0, interval, sc["damping_factor"], sc["damping_factor"], tol_f, tol_v, cpu_steps)
+        cpu_per_step = (time.perf_counter() - t0) / cpu_steps
+        cpu = {"seconds_extrapolated": cpu_per_step * n64, "per_step_s": cpu_per_step,
+               "cores": orc.lib().orc_num_threads(), "kind": "port",
+               "sample": f"{cpu_steps} steps of the same protocol, extrapolated to the FP64 leg's {n64}"}
+    return {
+        "workload": f"small defect disc, {sh.T} triangles, LCE lambda 1->0.9 over {K} steps, h=0.01; "
+                    f"force tol {tol_f} F_char, speed tol {tol_v} c_char, checks every {interval} steps",
+        "legs": legs,
+        "cpu_baseline": cpu,
+        "shapes": {"fp64": shape64, "mixed": shape_mixed},
+    }
+
+
+def bench_relax(args) -> dict:
+    from morphoshell_amd import autotune, workloads
+    from morphoshell_amd.shell import Context
+
+    sh = workloads.small_defect_disc()
+    x0, v0 = workloads.flat_noisy_state(sh)
+    space = dict(autotune.DEFAULT_SPACE, node_block=(64, 128), fused_node_block=(64, 128))
+    tuned = {}
+    for prec in ("fp64", "mixed"):
+        with Context(sh, precision=prec) as ctx:
+            ctx.set_state(0, x0, v0)
+            tuned[prec] = autotune.tune(ctx, steps=400, space=space, use_cache=False)
+            log(f"{prec} tuning: {tuned[prec]}")
+    shapes = {p: {k: t[k] for k in ("tri_block", "node_block", "tri_grid_per_cu", "node_grid_per_cu",
+                                     "graph_steps", "fused", "tile_chunks") if k in t} for p, t in tuned.items()}
+    res = c1_relaxation(args.relax_steps, shapes["fp64"], shapes["mixed"])
+    return {"metric": "time-to-equilibrium", "unit": "s", "higher_is_better": False,
+            "value": res["legs"]["fp64"]["seconds"], "config": {"workload": res["workload"]}, **res,
+            "tuning": tuned}
 
 
 def main() -> int:
@@ -919,7 +1000,12 @@ def main() -> int:
     ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax"])
     ap.add_argument("--spacing", type=float, default=0.002)
     ap.add_argument("--shells", type=int, default=512)
-    ap.add_argument("--relax-steps", type=int, default=20000)
+    ap.add_argument("--relax-steps", type=int, default=3_000_000,
+                    help="step bound of each time-to-equilibrium run (the small disc takes ~1M)")
+    ap.add_argument("--batch-steps", type=int, default=20000,
+                    help="batch workload: step bound of its bounded relaxation")
+    ap.add_argument("--no-relax", action="store_true",
+                    help="single workload: skip the small disc's time-to-equilibrium legs")
     ap.add_argument("--relax-large-steps", type=int, default=60000)
     ap.add_argument("--batch-tmin", type=int, default=500, help="batch_relax workload: smallest shell (triangles)")
     ap.add_argument("--batch-tmax", type=int, default=4000, help="batch_relax workload: largest shell (triangles)")
@@ -929,7 +1015,6 @@ def main() -> int:
                     help="batch_relax workload: also an FP64 run from states perturbed by 1e-10")
     ap.add_argument("--c5-spacing", type=float, default=0.001)
     ap.add_argument("--c5-steps", type=int, default=60000)
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
