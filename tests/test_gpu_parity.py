@@ -832,21 +832,34 @@ def test_wrapper_checks_array_sizes_and_shell_indices(sh):
         assert np.array_equal(ctx.get_state(0)[0], x)
 
 
+@pytest.mark.parametrize("tile_chunks", [1, 2])
+@pytest.mark.parametrize("grid_per_cu", [0, 2, 8])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_halted_shells_fused_active_tiles(precision):
+def test_halted_shells_fused_active_tiles(precision, grid_per_cu, tile_chunks):
     """With the fused form, after some shells halt the running shells' tiles
-    run from the active-tile list, with more tiles than CUs and with fewer.
-    The running shells' states equal, bit for bit, those of a context where
-    nothing halts."""
-    shells = [_shell(0.04, seed=i, h=0.02, kappa=0.3 - 0.05 * i) for i in range(24)]
+    run from the active-tile list, with a grid of one workgroup per tile
+    (grid_per_cu 0) or a persistent grid of 2 or 8 workgroups per CU that
+    strides over the list when it holds more tiles than the grid has
+    workgroups, and with tiles of one or two chunks.  The running
+    shells' states equal, bit for bit, those of a context where nothing halts
+    and that runs one workgroup per tile of one chunk."""
+    # 24 shells of 9.2k nodes: 144 tiles each at 64-node tiles of one chunk
+    # (72 at two chunks), so 3456 / 1728 tiles, 2880 / 1440 running after the
+    # first halts and 1728 / 864 after the second; a grid of 2 per CU (512
+    # workgroups on 256 CUs) strides over the list throughout, one of 8 per
+    # CU (2048) strides before the second halts at one chunk
+    shells = [_shell(0.02, seed=i, h=0.02, kappa=0.3 - 0.02 * i) for i in range(24)]
     states = [deformed_state(s, seed=300 + i) for i, s in enumerate(shells)]
     with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        a.set_tuning(node_block=64, tri_block=64, fused=1, node_grid_per_cu=grid_per_cu, tile_chunks=tile_chunks)
+        b.set_tuning(node_block=64, tri_block=64, fused=1, node_grid_per_cu=0, tile_chunks=1)
+        assert a.get_tuning()["node_grid_per_cu"] == grid_per_cu
+        assert a.get_tuning()["tile_chunks"] == tile_chunks
         for c in (a, b):
-            c.set_tuning(node_block=64, tri_block=64, fused=1)
             for i, st in enumerate(states):
                 c.set_state(i, *st)
         running = set(range(len(shells)))
-        for halting in (range(0, 4), range(4, 20)):  # ~720 tiles left running, then ~144
+        for halting in (range(0, 4), range(4, 12)):
             for i in halting:
                 a.relax_setup(i, steps_per_dial_phase=1, dial_spacing=1.0, dial_start=1.0, equil_interval=2,
                               tol_f=1e30, tol_v=1e30)
