@@ -117,6 +117,12 @@ def step_bytes(T: int, N: int, mixed: bool, uniform: bool) -> int:
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_grf_disc_1.8M.txt")
 BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_ragged_batch_512.txt")
+# The fused step's fastest shapes on the 1.8M-triangle disc, both on 512-node
+# tiles and a persistent grid of two workgroups per CU: FP64 as two chunks of
+# 256 nodes (256-thread workgroups, passes of 256 triangles, so two
+# workgroups fit a CU's LDS), mixed as one chunk of 512.
+LARGE_FUSED = {"fp64": dict(node_block=256, tile_chunks=2, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2),
+               "mixed": dict(node_block=512, tile_chunks=1, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2)}
 
 
 def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
@@ -228,7 +234,7 @@ def baseline_parity(sh, x, v, n: int = 20) -> dict:
     Eo = sysm.energies(x, v)
     parity["fp64_energy_rel_err"] = float(np.abs(E[:2] - Eo[:2]).max() / max(abs(Eo[:2]).max(), 1e-300))
     with Context(sh, precision="fp64") as ctx:
-        ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
+        ctx.set_tuning(**LARGE_FUSED["fp64"])
         ctx.set_state(0, x, v)
         ctx.step(n)
         xg = ctx.get_state(0)[0]
@@ -253,16 +259,14 @@ def bench_single(args) -> dict:
     log(f"T={T} N={N} mesh + set-up {t_setup:.1f}s")
     x, v = workloads.flat_noisy_state(sh)
     states = [(x, v)]
-    # the product form is the fused tile step (256-node tiles); the
-    # triangle + node launch form is timed beside it for reference
-    # the fused form at its fastest shape for this shell: 512-node tiles, two workgroups per CU
-    forms = {"fused": dict(node_block=512, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2),
-             "two_launch": dict(tri_block=256, node_block=256, fused=0, tri_grid_per_cu=0, node_grid_per_cu=0)}
+    # the product form is the fused tile step, at its fastest shape for this
+    # shell; the triangle + node launch form is timed beside it for reference
+    two_launch = dict(tri_block=256, node_block=256, fused=0, tri_grid_per_cu=0, node_grid_per_cu=0)
     res = {}
     for prec in args.precisions.split(","):
         with Context(sh, precision=prec) as ctx:
             for form in args.forms.split(","):
-                ctx.set_tuning(**forms[form])
+                ctx.set_tuning(**(LARGE_FUSED[prec] if form == "fused" else two_launch))
                 secs = time_steps(ctx, states, args.steps, args.warmup)
                 kt = kernel_times(ctx, states, min(args.steps, 50))
                 res[(prec, form)] = {"secs": secs, "kt": kt}
@@ -635,7 +639,7 @@ def bench_relax_large(args) -> dict:
     series, energy = {}, {}
     for prec in ("fp64", "mixed"):
         with Context(sh, precision=prec) as ctx:
-            ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
+            ctx.set_tuning(**LARGE_FUSED[prec])
             ctx.set_state(0, x0, v0)
             ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0,
                             equil_interval=interval, tol_f=1e-6, tol_v=1e-5)
@@ -702,7 +706,7 @@ def _relax_series(sh, precision, x, v, K, steps, chunk, from_ansatz=False):
         if from_ansatz:
             a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(0)
             ctx.set_zero_dial_geometry(0, a.reshape(-1), b.reshape(-1))
-        ctx.set_tuning(node_block=512, fused=1, node_grid_per_cu=2)
+        ctx.set_tuning(**LARGE_FUSED[precision])
         ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=1000,
                         tol_f=1e-6, tol_v=1e-5)
         ctx.synchronize()

@@ -272,6 +272,8 @@ int build_tile_records(ms_ctx* c) {
     return MS_OK;
 }
 
+bool has_lce(const ms_ctx* c);
+
 // Tile metadata of the fused step for the current node chunks (see TileData):
 // per tile the ascending triangles whose patch touches one of its nodes, the
 // halo nodes they reach, tile-local patch ids, and per pass the LDS positions
@@ -280,12 +282,7 @@ int build_tiles(ms_ctx* c) {
     free_tiles(c);
     const int T = c->T, N = c->N;
     const int B = c->tune.node_block;
-    // Passes of 2B triangles in FP64 (two workgroups per CU leave the LDS
-    // for it, and half the barriers measured 3% faster), B in mixed mode
-    // (four workgroups per CU).  Chosen when the tiles are built; a later
-    // precision switch keeps them (correct either way).  A multiple of 64:
-    // each wave's entries are one record block.
-    const int split = c->precision == MS_MIXED ? B : 2 * B;
+    const bool mixed = c->precision == MS_MIXED;
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
     std::vector<Chunk> nc(c->n_node_chunks);
@@ -305,17 +302,63 @@ int build_tiles(ms_ctx* c) {
             tiles.push_back(make_int4(nc[first].x, nc[last].y, s, first));
         }
     }
-    const int stride = npt * B + 1;
-    // pass count of the largest tile first (it sizes the offset table)
-    int max_passes = 1;
+    // per pass npt * B + 1 offsets, padded to an even count so that every
+    // tile's table starts on a 32-bit word (the kernel stages it by words)
+    const int stride = npt * B + 2;
+    // Each tile's triangles, and for the two pass lengths 2B and B (a
+    // multiple of 64: each wave's entries are one record block) the LDS a
+    // tile needs: its nodes and halo nodes, the largest pass's entries and
+    // the offset table.
     std::vector<std::vector<int>> tile_tris(tiles.size());
-    for (size_t ci = 0; ci < tiles.size(); ++ci) {
-        std::vector<int>& tl = tile_tris[ci];
-        for (long long e = off[tiles[ci].x]; e < off[tiles[ci].y]; ++e) tl.push_back(ent[e] / 6);
-        std::sort(tl.begin(), tl.end());
-        tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
-        max_passes = std::max(max_passes, (int)((tl.size() + split - 1) / split));
+    const int cand[2] = {2 * B, B};
+    int cand_entries[2] = {0, 0}, cand_passes[2] = {1, 1}, most_nodes = 0;
+    {
+        std::vector<int> hl, pc;
+        for (size_t ci = 0; ci < tiles.size(); ++ci) {
+            std::vector<int>& tl = tile_tris[ci];
+            const int n0 = tiles[ci].x, n1 = tiles[ci].y;
+            for (long long e = off[n0]; e < off[n1]; ++e) tl.push_back(ent[e] / 6);
+            std::sort(tl.begin(), tl.end());
+            tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+            hl.clear();
+            for (int t : tl)
+                for (int sl = 0; sl < 6; ++sl) {
+                    const int p = pid[(size_t)sl * T + t];
+                    if (p < n0 || p >= n1) hl.push_back(p);
+                }
+            std::sort(hl.begin(), hl.end());
+            most_nodes = std::max(most_nodes, n1 - n0 + (int)(std::unique(hl.begin(), hl.end()) - hl.begin()));
+            for (int k = 0; k < 2; ++k) {
+                const int sp = cand[k];
+                pc.assign((tl.size() + sp - 1) / sp, 0);
+                cand_passes[k] = std::max(cand_passes[k], (int)pc.size());
+                for (long long e = off[n0]; e < off[n1]; ++e)
+                    ++pc[(std::lower_bound(tl.begin(), tl.end(), ent[e] / 6) - tl.begin()) / sp];
+                for (int v : pc) cand_entries[k] = std::max(cand_entries[k], v);
+            }
+        }
     }
+    // Passes of 2B triangles (half the barriers of B; measured 3% faster in
+    // FP64 at one workgroup per CU) unless their LDS would cost a workgroup
+    // of those the kernel's registers allow per CU: 512 threads in FP64 (2
+    // waves per SIMD), 1024 in mixed mode (4; 768 with LCE shells).  FP64
+    // 256-thread blocks of two-chunk (512-node) tiles thus take B and run two
+    // workgroups per CU: 0.153 against 0.163 ms per step of the 1.8M-triangle
+    // disc at one 512-thread workgroup of 2B passes; mixed mode at 512-node
+    // tiles takes B (2B measured 28% slower).  Chosen when the tiles are
+    // built; a later precision switch keeps them (correct either way).
+    const int resident = std::max(1, (mixed ? (has_lce(c) ? 768 : 1024) : 512) / B);
+    int pick = 1;
+    {
+        TileData probe{};
+        probe.poff_stride = stride;
+        probe.max_nodes = most_nodes;
+        probe.max_entries = cand_entries[0];
+        probe.max_passes = cand_passes[0];
+        if (resident * (tile_lds_bytes(probe, mixed) + 1024) <= 160 * 1024) pick = 0;
+    }
+    const int split = cand[pick];
+    const int max_passes = cand_passes[pick];
     std::vector<int> toff(tiles.size() + 1, 0), tend(tiles.size(), 0), tris, hoff(tiles.size() + 1, 0), halo;
     std::vector<unsigned short> lpid, dst, poff(tiles.size() * (size_t)max_passes * stride, 0);
     int max_entries = 0, max_nodes = 0;

@@ -169,7 +169,7 @@ struct TileData {
     const int* halo_off;           // ntiles + 1
     const int* halo;               // global ids of halo nodes
     const unsigned short* poff;    // max_passes * poff_stride per tile
-    int poff_stride;               // tile nodes + 1
+    int poff_stride;               // chunk nodes per tile + 2 (even: tables start on 32-bit words)
     int pass_split;                // entries per pass (B or 2B)
     int max_passes;
     int max_nodes;                 // max over tiles of own + halo nodes
@@ -184,10 +184,12 @@ struct TileData {
 void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp, int slot,
                         const int* tris);
 // Dynamic LDS of a tile launch: positions of the tile's own and halo nodes
-// (24 B each), in mixed mode the own nodes' running sums (24 B each), and one
-// pass of contributions (3 values per entry).
+// (24 B each), in mixed mode the own nodes' running sums (24 B each), one
+// pass of contributions (3 values per entry), and the tile's per-pass entry
+// offsets (16-bit, max_passes rows of poff_stride).
 inline int tile_lds_bytes(const TileData& tl, bool mixed) {
-    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8);
+    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8) +
+           ((2 * tl.max_passes * tl.poff_stride + 7) & ~7);
 }
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.
 hipError_t prepare_tile_kernels(int lds_bytes_fp64, int lds_bytes_mixed);

@@ -34,6 +34,25 @@ struct TileKArgs {
     StepFlags fl;
 };
 
+// Block-uniform tables and the shell record, read through the constant
+// address space so that they are scalar loads: those bypass the vector memory
+// queue, which the record streams keep full, so the chain tile -> shell ->
+// dial factor at the start of a tile does not wait behind them.  The tables
+// are written only at set-up and the fields read here of the shell record
+// only by the bookkeeping launch between steps (the scalar cache is
+// invalidated at every launch); the atomically set failure flag is not read.
+template <typename T>
+__device__ __forceinline__ T kload(const T* p, long long i) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    typedef const __attribute__((address_space(4))) unsigned KWord;
+    KWord* src = (KWord*)(p + i);
+    T v;
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(T) / 4); ++w) d[w] = src[w];
+    return v;
+}
+
 template <int NV>
 __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
     __shared__ double sv[NV][16];
@@ -79,7 +98,9 @@ __device__ __forceinline__ void block_argmax(double v[NV], int idx[NV]) {
 // pass in LDS so that they hold no registers while triangles are evaluated;
 // the FP64 kernel keeps them in registers, where its larger LDS tile would
 // otherwise cost a workgroup per CU), then contributions (3 CT per pass
-// entry).
+// entry), then the tile's per-pass entry offsets (staged with the positions:
+// a global load in a pass's sums would wait, by the in-order count of
+// outstanding vector loads, for the next entries' records already in flight).
 // Waves per SIMD asked of the compiler (their LDS tiles allow them): 2 in
 // FP64 and 4 in mixed mode (3 with the LCE protocol).  Each triangle's record
 // is loaded whole before its first use; at 2 waves the FP64 kernel keeps
@@ -96,33 +117,114 @@ void tile_step_kernel(TileKArgs k) {
     double* lpos = smem;
     double* lsum = smem + 3 * (size_t)k.tl.max_nodes;
     CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)stride : lsum);
+    unsigned short* lpo = reinterpret_cast<unsigned short*>(lcon + 3 * (size_t)k.tl.max_entries);
     const int nt = k.tl.active ? k.tl.nactive : k.tl.ntiles;
     for (int vi = blockIdx.x; vi < nt; vi += gridDim.x) {
         const int vr = xcd_remap(vi, nt);
-        const int ci = k.tl.active ? uni(k.tl.active[vr]) : vr;
+        const int ci = k.tl.active ? kload(k.tl.active, vr) : vr;
         // nodes [ch.x, ch.y) of shell ch.z; chunk ch.w + r holds nodes from ch.x + r * B
-        const int4 ch = make_int4(uni(k.tl.tiles[ci].x), uni(k.tl.tiles[ci].y), uni(k.tl.tiles[ci].z),
-                                  uni(k.tl.tiles[ci].w));
-        const ShellDev& sh = k.shells[ch.z];
+        const int4 ch = kload(k.tl.tiles, ci);
+        const ShellDev sh = kload(k.shells, ch.z);
         if (uni(sh.halted)) continue;  // block-uniform
         const bool do_check = CHECK || uni(sh.check_due);
         const int par = uni(sh.par);
         const double* X = par ? k.b.x[1] : k.b.x[0];
         const TriParams tp = tri_params<UNI>(sh);
         const int nn = ch.y - ch.x;
-        const int h0 = uni(k.tl.halo_off[ci]), h1 = uni(k.tl.halo_off[ci + 1]);
-        // 1. positions of the tile's nodes and its halo nodes
+        const int h0 = kload(k.tl.halo_off, ci), h1 = kload(k.tl.halo_off, ci + 1);
+        const int j0 = kload(k.tl.tri_off, ci), j1 = kload(k.tl.tri_end, ci);
+        const int split = k.tl.pass_split;
+        const int npass = (j1 - j0 + split - 1) / split;
+        const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * stride;
+        // the record block of entry j (a wave's 64 entries start at a multiple of 64)
+        auto view = [&](int j) {
+            const size_t blk = (size_t)uni(j >> 6) * (REC_SLOTS * 64);
+            const double* r64 = k.tl.rec64 + blk;
+            const float* r32 = k.tl.rec32 + blk;
+            TriData rv{};
+            rv.T = 64;
+            rv.G = r64 + RS_G * 64;
+            rv.H = r64 + RS_H * 64;
+            rv.ainfo = r64 + RS_AI * 64;
+            rv.binfo = r64 + RS_BI * 64;
+            rv.h = r64 + RS_THICK * 64;
+            rv.mu = r64 + RS_MU * 64;
+            rv.area = r64 + RS_AREA * 64;
+            rv.a0 = r64 + RS_A0 * 64;
+            rv.b0 = r64 + RS_B0 * 64;
+            rv.G32 = r32 + RS_G * 64;
+            rv.H32 = r32 + RS_H * 64;
+            rv.hmuA32 = r32 + RS_HMUA * 64;
+            return rv;
+        };
+        // six 16-bit values of entry j (read as three 32-bit words) from
+        // stream w: its tile-local patch ids (lpid) or the pass-local LDS
+        // positions of its contributions (dst)
+        auto words = [&](const unsigned short* w, int j, unsigned q[3]) {
+            // opaque to loop strength reduction: the loads keep one 32-bit
+            // lane offset against scalar bases instead of a 64-bit induction
+            // pointer per stream
+            unsigned t = (unsigned)j;
+            asm volatile("" : "+v"(t));
+            const unsigned* p = reinterpret_cast<const unsigned*>(w);
+            q[0] = ld_stream<!MIXED>(p, 3 * t);
+            q[1] = ld_stream<!MIXED>(p, 3 * t + 1);
+            q[2] = ld_stream<!MIXED>(p, 3 * t + 2);
+        };
+        auto unpack = [](const unsigned q[3], unsigned v[6]) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                v[2 * s] = q[s] & 0xFFFFu;
+                v[2 * s + 1] = q[s] >> 16;
+            }
+        };
+        // an entry's contributions to the tile's nodes, at their pass-local LDS positions
+        auto store = [&](const unsigned dw[3], const CT out[18]) {
+            unsigned dq[6];
+            unpack(dw, dq);
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                const unsigned q = dq[s];
+                if (q != 0xFFFFu) {
+                    lcon[3 * q] = out[3 * s];
+                    lcon[3 * q + 1] = out[3 * s + 1];
+                    lcon[3 * q + 2] = out[3 * s + 2];
+                }
+            }
+        };
+        // FP64: the thread's entries are j0 + tid + m B (m = 0, 1, ...) across
+        // all passes, and each one's record, ids and LDS positions are loaded
+        // as soon as the previous entry's contributions are in LDS: at the end
+        // of a pass the loads are in flight through the barriers and the node
+        // sums, and the first entry's are in flight with the position staging
+        // below.  (Loading them during the previous entry's mechanics, or
+        // before its LDS store, needs more than the 256 registers of 2 waves
+        // per SIMD and spills.)  The LDS positions come with the ids, so a
+        // store never waits on a global load.
+        int jn = j0 + (int)threadIdx.x;
+        RecF64 nrec;
+        unsigned nlw[3], ndw[3];
+        auto fetch = [&](int j) {
+            words(k.tl.lpid, j, nlw);
+            words(k.tl.dst, j, ndw);
+            load_rec_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, nrec);
+        };
+        if (!MIXED && jn < j1) fetch(jn);
+        // 1. positions of the tile's nodes and its halo nodes, and the tile's
+        // per-pass entry offsets
         for (int i = threadIdx.x; i < nn + (h1 - h0); i += B) {
             const unsigned g = i < nn ? (unsigned)(ch.x + i) : (unsigned)k.tl.halo[h0 + i - nn];
             lpos[3 * i] = ld(X, g);
             lpos[3 * i + 1] = ld(X + N, g);
             lpos[3 * i + 2] = ld(X + 2 * (size_t)N, g);
         }
+        {
+            // by 32-bit words: the table's stride is even
+            const unsigned* pw = reinterpret_cast<const unsigned*>(poff);
+            unsigned* lw = reinterpret_cast<unsigned*>(lpo);
+            for (int i = threadIdx.x; i < npass * (stride >> 1); i += B) lw[i] = pw[i];
+        }
         __syncthreads();
-        const int j0 = uni(k.tl.tri_off[ci]), j1 = uni(k.tl.tri_end[ci]);
-        const int split = k.tl.pass_split;
-        const int npass = (j1 - j0 + split - 1) / split;
-        const unsigned short* poff = k.tl.poff + (size_t)ci * k.tl.max_passes * stride;
         double fx[MAX_TILE_CHUNKS], fy[MAX_TILE_CHUNKS], fz[MAX_TILE_CHUNKS];
 #pragma unroll
         for (int r = 0; r < MAX_TILE_CHUNKS; ++r) fx[r] = fy[r] = fz[r] = 0.0;
@@ -130,86 +232,32 @@ void tile_step_kernel(TileKArgs k) {
         for (int pass = 0; pass < npass; ++pass) {
             // 2. this pass's triangles into LDS
             const int ja = j0 + pass * split, jb = min(j1, ja + split);
-            // the record block of entry j (a wave's 64 entries start at a multiple of 64)
-            auto view = [&](int j) {
-                const size_t blk = (size_t)uni(j >> 6) * (REC_SLOTS * 64);
-                const double* r64 = k.tl.rec64 + blk;
-                const float* r32 = k.tl.rec32 + blk;
-                TriData rv{};
-                rv.T = 64;
-                rv.G = r64 + RS_G * 64;
-                rv.H = r64 + RS_H * 64;
-                rv.ainfo = r64 + RS_AI * 64;
-                rv.binfo = r64 + RS_BI * 64;
-                rv.h = r64 + RS_THICK * 64;
-                rv.mu = r64 + RS_MU * 64;
-                rv.area = r64 + RS_AREA * 64;
-                rv.a0 = r64 + RS_A0 * 64;
-                rv.b0 = r64 + RS_B0 * 64;
-                rv.G32 = r32 + RS_G * 64;
-                rv.H32 = r32 + RS_H * 64;
-                rv.hmuA32 = r32 + RS_HMUA * 64;
-                return rv;
-            };
-            // entry j's tile-local patch ids (six 16-bit ids read as three 32-bit words)
-            auto patch = [&](int j, unsigned lp[6]) {
-                // opaque to loop strength reduction: the loads keep one 32-bit
-                // lane offset against scalar bases instead of a 64-bit induction
-                // pointer per stream
-                unsigned t = (unsigned)j;
-                asm volatile("" : "+v"(t));
-                const unsigned* lw = reinterpret_cast<const unsigned*>(k.tl.lpid);
-                const unsigned w0 = ld_stream<!MIXED>(lw, 3 * t), w1 = ld_stream<!MIXED>(lw, 3 * t + 1),
-                               w2 = ld_stream<!MIXED>(lw, 3 * t + 2);
-                lp[0] = w0 & 0xFFFFu;
-                lp[1] = w0 >> 16;
-                lp[2] = w1 & 0xFFFFu;
-                lp[3] = w1 >> 16;
-                lp[4] = w2 & 0xFFFFu;
-                lp[5] = w2 >> 16;
-            };
-            // entry j's contributions to the tile's nodes, at their pass-local LDS positions
-            auto store = [&](int j, const CT out[18]) {
-                unsigned t = (unsigned)j;
-                asm volatile("" : "+v"(t));
-                const unsigned* dw = reinterpret_cast<const unsigned*>(k.tl.dst);
-                const unsigned d0 = ld_stream<!MIXED>(dw, 3 * t), d1 = ld_stream<!MIXED>(dw, 3 * t + 1),
-                               d2 = ld_stream<!MIXED>(dw, 3 * t + 2);
-                const unsigned dq[6] = {d0 & 0xFFFFu, d0 >> 16, d1 & 0xFFFFu, d1 >> 16, d2 & 0xFFFFu, d2 >> 16};
-#pragma unroll
-                for (int s = 0; s < 6; ++s) {
-                    const unsigned q = dq[s];
-                    if (q != 0xFFFFu) {
-                        lcon[3 * q] = out[3 * s];
-                        lcon[3 * q + 1] = out[3 * s + 1];
-                        lcon[3 * q + 2] = out[3 * s + 2];
-                    }
-                }
-            };
             if constexpr (MIXED) {
                 for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
-                    unsigned lp[6];
-                    patch(j, lp);
+                    unsigned lw[3], lp[6], dw[3];
+                    words(k.tl.lpid, j, lw);
+                    unpack(lw, lp);
                     const TilePos pos{lpos, lp};
                     float out[18];
                     tri_forces_mixed<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, pos, out);
-                    store(j, reinterpret_cast<const CT*>(out));
+                    // the mixed kernel, at its 128-register limit, loads the
+                    // LDS positions at the store (loaded with the ids: 5% slower)
+                    words(k.tl.dst, j, dw);
+                    store(dw, reinterpret_cast<const CT*>(out));
                 }
             } else {
-                // one entry at a time: the next entry's record loaded before this
-                // one's mechanics needs all 256 registers of 2 waves per SIMD and
-                // spills (40 B per lane), and measured 24% slower
-                for (int j = ja + (int)threadIdx.x; j < jb; j += B) {
-                    unsigned lp[6];
-                    patch(j, lp);
+                for (; jn < jb; jn += B) {
+                    unsigned lp[6], dw[3] = {ndw[0], ndw[1], ndw[2]};
+                    unpack(nlw, lp);
                     double out[18];
-                    tri_forces_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, TilePos{lpos, lp}, out);
-                    store(j, reinterpret_cast<const CT*>(out));
+                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, out);
+                    store(dw, reinterpret_cast<const CT*>(out));
+                    if (jn + B < j1) fetch(jn + B);
                 }
             }
             __syncthreads();
             // 3. each node adds this pass's entries, in CSR order
-            const unsigned short* po = poff + pass * stride;
+            const unsigned short* po = lpo + pass * stride;
 #pragma unroll
             for (int r = 0; r < MAX_TILE_CHUNKS; ++r) {
                 const int i = (int)threadIdx.x + r * B;

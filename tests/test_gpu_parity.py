@@ -788,12 +788,31 @@ def test_device_info_reports_the_device():
     assert short.raw[3:4] == b"\0" and short.value == name.value[:3]
 
 
-def test_refused_tile_shape_leaves_a_working_two_launch_context(sh):
-    """A fused shape whose tiles do not fit the LDS (1024-node FP64 tiles
-    here) is refused with an error; the context falls back to the two-launch
-    form at the new block size and steps exactly as a context set to it."""
+def _relabelled_shell(seed=5):
+    """A 9.2k-node disc whose node labels are randomly permuted: a tile (a
+    run of consecutive labels) is scattered over the whole disc, so its
+    triangles reach most of the nodes (1024-node tiles: 8289 of 9214, whose
+    positions alone need 199 KB of LDS)."""
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    m = grf_disc(0.02, seed=seed, h=0.02, kappa=0.2)
+    perm = np.random.default_rng(seed).permutation(m.num_nodes)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    m.points = m.points[perm]
+    m.tris = inv[m.tris]
+    m.constrained = m.constrained[perm]
+    return prepare_shell(m, Settings())
+
+
+def test_refused_tile_shape_leaves_a_working_two_launch_context():
+    """A fused shape whose tiles do not fit the LDS (1024-node tiles of a
+    mesh with scattered node labels) is refused with an error; the context
+    falls back to the two-launch form at the new block size and steps exactly
+    as a context set to it."""
     from morphoshell_amd.native import NativeError
 
+    sh = _relabelled_shell()
     x, v = deformed_state(sh, seed=41)
     with _ctx(sh) as a, _ctx(sh) as b:
         a.set_tuning(node_block=256, fused=1)
@@ -806,6 +825,27 @@ def test_refused_tile_shape_leaves_a_working_two_launch_context(sh):
             c.set_state(0, x, v)
             c.step(20)
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+
+
+@pytest.mark.parametrize("node_block,tile_chunks", [(512, 2), (256, 2)])
+def test_two_chunk_fp64_tiles_take_short_passes_and_match_two_launch(sh, node_block, tile_chunks):
+    """FP64 tiles of two chunks run passes of B triangles (2B passes would
+    not leave the LDS for the workgroups per CU the registers allow; at 512
+    threads they would not fit at all) and step bit-identically to the
+    two-launch form, the equilibrium check included."""
+    x, v = deformed_state(sh, seed=43)
+    with _ctx(sh) as a, _ctx(sh) as b:
+        a.set_tuning(node_block=node_block, tile_chunks=tile_chunks, fused=1)
+        assert a.get_tuning()["fused"] == 1
+        b.set_tuning(node_block=node_block, fused=0)
+        for c in (a, b):
+            c.set_state(0, x, v)
+        ra = a.step(24, equil_every=8)
+        rb = b.step(24, equil_every=8)
+        assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
+        assert np.array_equal(a.get_state(0)[1], b.get_state(0)[1])
+        assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
+        assert ra[0]["max_speed_ratio"] == rb[0]["max_speed_ratio"]
 
 
 def test_wrapper_checks_array_sizes_and_shell_indices(sh):
