@@ -465,6 +465,15 @@ int ms_nn_set_params(ms_nn* nn, const float* params);
 int ms_nn_set_graph(ms_nn* nn, const ms_nn_graph_desc* g);
 /* Predictions for the current graph (num_nodes x 3); out may be NULL. */
 int ms_nn_forward(ms_nn* nn, float* out);
+/* Inference without the training state: the predictions (num_nodes x 3)
+ * for graph g, equal bit for bit to ms_nn_set_graph + ms_nn_forward.  Each
+ * half-layer is one fused launch (the concatenated rows gathered into the
+ * first GEMM's A tile, the in-edge sums in CSR order, both weight matrices
+ * in LDS, bias and SiLU in the epilogues); device memory is (2 E + N) x width
+ * floats plus the graph, against some 36 edge-sized training buffers.  The
+ * graph set by ms_nn_set_graph is kept.  Width 32 or 64.  Its output is the
+ * start state that replaces the ansatz of move_nodes_to_ansatz_positions.cpp:26-39. */
+int ms_nn_predict(ms_nn* nn, const ms_nn_graph_desc* g, float* out);
 /* After ms_nn_forward: loss = sum over graphs of the smaller of the squared
  * errors against target and against target reflected in z, / (3 num_nodes),
  * and its gradient w.r.t. every parameter (grads may be NULL). */

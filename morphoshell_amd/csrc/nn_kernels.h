@@ -23,7 +23,34 @@ struct GemmArgs {
     int accumulate;
     int ksplit;   // > 1: split K over blocks; needs a partial buffer of ksplit*M*N floats and ldc == N
     int kchunk;   // set by gemm()
+    int silu;     // 1: C = silu(A B + bias) (ksplit 1, no accumulation)
 };
+
+// One half of a message-passing layer for inference, in one launch over
+// tiles of 64 rows (rows = edges or nodes):
+//   edge: m = silu([h_dst, h_src, e] W1 + b1) W2 + b2;  e += m
+//   node: h += silu([h, sum of m over the node's in-edges] W1 + b1) W2 + b2
+// The concatenated rows are gathered straight into the first GEMM's LDS A
+// tile (in-edge sums in CSR order), both weight matrices sit in LDS as
+// bfloat16, and bias and SiLU run in the GEMMs' epilogues, so no E x 3W or
+// N x 2W intermediate exists.  Operand roundings and the MFMA sequence are
+// those of gemm(), so the result equals the unfused forward bit for bit.
+struct MlpLayerArgs {
+    const float* W1;  // K1 x W row-major (K1 = 3W edge, 2W node)
+    const float* b1;  // W
+    const float* W2;  // W x W
+    const float* b2;  // W
+    int M;            // rows: edges or nodes
+    float* h;         // N x W
+    float* e;         // E x W (edge)
+    float* m;         // E x W: written by the edge half, read by the node half
+    const int* src;   // E (edge)
+    const int* dst;   // E (edge)
+    const int* in_off;  // N + 1, edges sorted by destination (node)
+    int grid;         // persistent blocks
+};
+// hipErrorInvalidValue for a width other than 32 or 64.
+hipError_t mlp_layer(hipStream_t s, int W, bool edge, const MlpLayerArgs& a);
 
 void gemm(hipStream_t s, const GemmArgs& a, float* partials);
 void silu(hipStream_t s, const float* x, float* y, long long n);

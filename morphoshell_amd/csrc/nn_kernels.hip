@@ -1,6 +1,6 @@
 // Kernels of the mesh message-passing network (learned warm start).
 //
-// GEMMs run on the matrix cores: v_mfma_f32_16x16x16_bf16 with operands
+// GEMMs run on the matrix cores: v_mfma_f32_16x16x32_bf16 with operands
 // rounded to bfloat16 (round-to-nearest-even) as they are staged into LDS,
 // FP32 accumulation, FP32 activations and master weights in memory.  Every
 // reduction (split-K GEMM partials, bias gradients, edge aggregation, loss)
@@ -29,16 +29,32 @@ __device__ __forceinline__ short f2bf(float f) {
     return (short)(u >> 16);
 }
 
-constexpr int BM = 64, BN = 64, BK = 16;
+__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
+
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// gfx950's v_mfma_f32_16x16x32_bf16: D(16x16) += A(16x32) B(32x16).  Lane l
+// holds A's row l % 16 and B's column l % 16, each at k = 8 (l / 16) .. +7;
+// D's column l % 16 at rows 4 (l / 16) .. +3.
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
+                                                    0, 0);
+}
+
+constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int LDK = BK + 8;  // LDS row of 40 bf16 (80 B): 16-byte aligned rows, no bank conflicts between them
 
 // C[z] = A[:, kz] @ B[kz, :] for the K-range of slice z (or the whole K when
 // ksplit == 1), with A(i,k) = A[i*sa0 + k*sa1], B(k,j) = B[k*sb0 + j*sb1],
 // C(i,j) = C[z*M*N + i*ldc + j].  Epilogue: + bias[j] (slice 0 only) and
 // optional accumulation into C.  256 threads: wave w owns rows 16w..16w+15
-// of the 64x64 tile and four 16x16 MFMA accumulators along N.
+// of the 64x64 tile and four 16x16 MFMA accumulators along N.  Both tiles
+// are staged k-contiguous (B transposed) so that each lane reads its eight
+// operand values as one 16-byte LDS load.
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ short As[BM][BK + 4];
-    __shared__ short Bs[BK][BN + 4];
+    __shared__ __attribute__((aligned(16))) short As[BM][LDK];
+    __shared__ __attribute__((aligned(16))) short Bt[BN][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // row tiles on x (up to 2^31 - 1 blocks: edge GEMMs of large meshes have
     // millions of rows), column tiles on y
@@ -50,9 +66,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        // stage A (64 x 16) and B (16 x 64): 4 elements per thread each
+        // stage A (64 x 32) and B^T (64 x 32): 8 elements per thread each
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < 8; ++r) {
             const int idx = tid + r * 256;
             const int ai = idx / BK, ak = idx % BK;
             const int gi = m0 + ai, gk = k0 + ak;
@@ -63,25 +79,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             const int hk = k0 + bk, gj = n0 + bj;
             float bv = 0.f;
             if (hk < kend && gj < g.N) bv = g.B[(size_t)hk * g.sb0 + (size_t)gj * g.sb1];
-            Bs[bk][bj] = f2bf(bv);
+            Bt[bj][bk] = f2bf(bv);
         }
         __syncthreads();
-        const int row = wave * 16 + (lane & 15);
-        const int kq = (lane >> 4) * 4;
-        v4s a;
-        a.x = As[row][kq];
-        a.y = As[row][kq + 1];
-        a.z = As[row][kq + 2];
-        a.w = As[row][kq + 3];
+        const int kq = (lane >> 4) * 8;
+        const v8s a = *reinterpret_cast<const v8s*>(&As[wave * 16 + (lane & 15)][kq]);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int col = t * 16 + (lane & 15);
-            v4s b;
-            b.x = Bs[kq][col];
-            b.y = Bs[kq + 1][col];
-            b.z = Bs[kq + 2][col];
-            b.w = Bs[kq + 3][col];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            const v8s b = *reinterpret_cast<const v8s*>(&Bt[t * 16 + (lane & 15)][kq]);
+            acc[t] = mfma32(a, b, acc[t]);
         }
         __syncthreads();
     }
@@ -97,9 +103,125 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             const int rowg = m0 + wave * 16 + (lane >> 4) * 4 + r;
             if (rowg >= g.M) continue;
             float v = acc[t][r] + bias;
+            if (g.silu) v = v * sigm(v);
             float* dst = C + (size_t)rowg * g.ldc + col;
             *dst = g.accumulate ? *dst + v : v;
         }
+    }
+}
+
+// LDS layout of mlp_layer_kernel (bfloat16 bits, rows padded by 8 so that
+// each lane's eight operand values are one aligned 16-byte load):
+//   W1t[W][K1 + 8]  W1 transposed,  W2t[W][W + 8],  As[64][K1 + 8],  Ts[64][W + 8].
+template <int W, bool EDGE>
+constexpr int mlp_lds_bytes() {
+    constexpr int K1 = EDGE ? 3 * W : 2 * W;
+    return 2 * (W * (K1 + 8) + W * (W + 8) + 64 * (K1 + 8) + 64 * (W + 8));
+}
+
+// One wave's 16 rows of C (16 x W) = A (16 x K, k-contiguous rows of stride
+// lda) B, with B given transposed (W rows of stride ldb): the MFMA sequence of
+// gemm_kernel (k ascending in steps of 32, 16-column blocks).
+template <int W, int K>
+__device__ __forceinline__ void wave_gemm(const short* A, int lda, const short* Bt, int ldb, v4f acc[W / 16]) {
+    const int lane = threadIdx.x & 63, kq = (lane >> 4) * 8;
+#pragma unroll
+    for (int t = 0; t < W / 16; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const v8s a = *reinterpret_cast<const v8s*>(A + (lane & 15) * lda + k0 + kq);
+#pragma unroll
+        for (int t = 0; t < W / 16; ++t) {
+            const v8s b = *reinterpret_cast<const v8s*>(Bt + (t * 16 + (lane & 15)) * ldb + k0 + kq);
+            acc[t] = mfma32(a, b, acc[t]);
+        }
+    }
+}
+
+template <int W, bool EDGE>
+__global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
+    constexpr int K1 = EDGE ? 3 * W : 2 * W;
+    constexpr int L1 = K1 + 8, L2 = W + 8;
+    extern __shared__ __attribute__((aligned(16))) short smem_mlp[];
+    short* W1t = smem_mlp;
+    short* W2t = W1t + W * L1;
+    short* As = W2t + W * L2;
+    short* Ts = As + 64 * L1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // weights, once per block: read k-major (coalesced over the output index), stored transposed
+    for (int i = tid; i < K1 * W; i += 256) W1t[(i % W) * L1 + i / W] = f2bf(a.W1[i]);
+    for (int i = tid; i < W * W; i += 256) W2t[(i % W) * L2 + i / W] = f2bf(a.W2[i]);
+    const int ntiles = (a.M + 63) / 64;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int r0 = tile * 64;
+        // A tile: 64 rows of K1 values, four at a time
+        constexpr int Q = K1 / 4, QW = W / 4;
+        for (int i = tid; i < 64 * Q; i += 256) {
+            const int r = i / Q, q = i % Q, row = r0 + r;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (row < a.M) {
+                const int seg = q / QW, c = (q % QW) * 4;
+                if (EDGE) {
+                    const float* p = seg == 0   ? a.h + (size_t)a.dst[row] * W
+                                     : seg == 1 ? a.h + (size_t)a.src[row] * W
+                                                : a.e + (size_t)row * W;
+                    v = *reinterpret_cast<const float4*>(p + c);
+                } else if (seg == 0) {
+                    v = *reinterpret_cast<const float4*>(a.h + (size_t)row * W + c);
+                } else {
+                    // the node's in-edge sum, edge by edge in CSR order
+                    for (int e = a.in_off[row]; e < a.in_off[row + 1]; ++e) {
+                        const float4 x = *reinterpret_cast<const float4*>(a.m + (size_t)e * W + c);
+                        v.x += x.x;
+                        v.y += x.y;
+                        v.z += x.z;
+                        v.w += x.w;
+                    }
+                }
+            }
+            short* d = As + r * L1 + q * 4;
+            d[0] = f2bf(v.x);
+            d[1] = f2bf(v.y);
+            d[2] = f2bf(v.z);
+            d[3] = f2bf(v.w);
+        }
+        __syncthreads();
+        v4f acc[W / 16];
+        wave_gemm<W, K1>(As + wave * 16 * L1, L1, W1t, L1, acc);
+        // bias, SiLU, rounded for the second GEMM (D: column lane & 15 of rows 4 (lane >> 4) + r)
+#pragma unroll
+        for (int t = 0; t < W / 16; ++t) {
+            const int col = t * 16 + (lane & 15);
+            const float b = a.b1[col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = acc[t][r] + b;
+                v = v * sigm(v);
+                Ts[(wave * 16 + (lane >> 4) * 4 + r) * L2 + col] = f2bf(v);
+            }
+        }
+        __syncthreads();
+        wave_gemm<W, W>(Ts + wave * 16 * L2, L2, W2t, L2, acc);
+#pragma unroll
+        for (int t = 0; t < W / 16; ++t) {
+            const int col = t * 16 + (lane & 15);
+            const float b = a.b2[col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + wave * 16 + (lane >> 4) * 4 + r;
+                if (row >= a.M) continue;
+                const float v = acc[t][r] + b;
+                if (EDGE) {
+                    a.m[(size_t)row * W + col] = v;
+                    float* ep = a.e + (size_t)row * W + col;
+                    *ep = *ep + v;
+                } else {
+                    float* hp = a.h + (size_t)row * W + col;
+                    *hp = *hp + v;
+                }
+            }
+        }
+        __syncthreads();  // the next tile restages As and Ts
     }
 }
 
@@ -112,8 +234,6 @@ __global__ void sum_partials_kernel(const float* part, int nz, long long n, floa
         out[i] = accumulate ? out[i] + s : s;
     }
 }
-
-__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
 
 __global__ void silu_kernel(const float* x, float* y, long long n) {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -292,6 +412,7 @@ void gemm(hipStream_t s, const GemmArgs& a0, float* partials) {
     p.C = partials;
     p.ldc = a.N;
     p.accumulate = 0;
+    p.silu = 0;  // silu is for unsplit GEMMs only
     hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, p);
     // partials are dense M x N; the destination has ldc == N in every split use
     const long long n = (long long)a.M * a.N;
@@ -301,6 +422,27 @@ void gemm(hipStream_t s, const GemmArgs& a0, float* partials) {
 
 void silu(hipStream_t s, const float* x, float* y, long long n) {
     hipLaunchKernelGGL(silu_kernel, dim3(blocks(n)), dim3(256), 0, s, x, y, n);
+}
+
+template <int W, bool EDGE>
+hipError_t mlp_layer_launch(hipStream_t s, const MlpLayerArgs& a) {
+    constexpr int lds = mlp_lds_bytes<W, EDGE>();
+    static const hipError_t attr =
+        lds > 65536 ? hipFuncSetAttribute((const void*)mlp_layer_kernel<W, EDGE>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds)
+                    : hipSuccess;
+    if (attr != hipSuccess) return attr;
+    const int ntiles = (a.M + 63) / 64;
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((mlp_layer_kernel<W, EDGE>), dim3(std::max(1, std::min(a.grid, ntiles))), dim3(256), lds, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t mlp_layer(hipStream_t s, int W, bool edge, const MlpLayerArgs& a) {
+    if (W == 64) return edge ? mlp_layer_launch<64, true>(s, a) : mlp_layer_launch<64, false>(s, a);
+    if (W == 32) return edge ? mlp_layer_launch<32, true>(s, a) : mlp_layer_launch<32, false>(s, a);
+    return hipErrorInvalidValue;
 }
 void silu_bwd(hipStream_t s, const float* x, const float* dy, float* dx, long long n) {
     hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks(n)), dim3(256), 0, s, x, dy, dx, n);

@@ -76,6 +76,10 @@ struct ms_nn {
     DBuf<float> pre_n, pre_e, d1pre, d1, out;
     DBuf<float> dH, dE, dtmp_e, dtmp_e2, dtmp_e3, dtmp_n, dtmp_n2, dtmp_n3, dout, tgt, terms;
     bool have_forward = false;
+    // inference (ms_nn_predict): its own graph and three state buffers
+    DBuf<int> p_src, p_dst, p_inoff;
+    DBuf<float> p_xn, p_xe, p_h, p_e, p_m, p_out;
+    int num_cus = 0;
 };
 
 namespace {
@@ -98,8 +102,8 @@ Lin make_lin(size_t& off, int in, int out) {
     return l;
 }
 
-// Y[M x out] = X[M x in] @ W + b
-void linear_fwd(ms_nn* m, const Lin& L, const float* X, int M, float* Y) {
+// Y[M x out] = X[M x in] @ W + b, or silu of it
+void linear_fwd(ms_nn* m, const Lin& L, const float* X, int M, float* Y, bool silu = false) {
     nn::GemmArgs g{};
     g.A = X;
     g.sa0 = L.in;
@@ -114,7 +118,31 @@ void linear_fwd(ms_nn* m, const Lin& L, const float* X, int M, float* Y) {
     g.N = L.out;
     g.K = L.in;
     g.ksplit = 1;
+    g.silu = silu ? 1 : 0;
     nn::gemm(m->stream, g, nullptr);
+}
+
+// The checks of a graph description, and its destination CSR.
+int check_graph(ms_nn* m, const ms_nn_graph_desc* g, const char* who, std::vector<int>& in_off) {
+    if (!g || g->num_graphs <= 0 || g->num_nodes <= 0 || g->num_edges <= 0 || !g->node_offsets || !g->edge_src ||
+        !g->edge_dst || !g->node_feat || !g->edge_feat)
+        return nfail(m, MS_ERR_ARG, fmt("%s: bad arguments", who));
+    const int G = g->num_graphs, N = g->num_nodes, E = g->num_edges;
+    if (g->node_offsets[0] != 0 || g->node_offsets[G] != N)
+        return nfail(m, MS_ERR_ARG, fmt("%s: node offsets must run from 0 to num_nodes", who));
+    for (int q = 0; q < G; ++q)
+        if (g->node_offsets[q + 1] < g->node_offsets[q])
+            return nfail(m, MS_ERR_ARG, fmt("%s: node offsets must not decrease", who));
+    in_off.assign(N + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        const int s = g->edge_src[e], d = g->edge_dst[e];
+        if (s < 0 || s >= N || d < 0 || d >= N) return nfail(m, MS_ERR_ARG, fmt("%s: edge node out of range", who));
+        if (e > 0 && d < g->edge_dst[e - 1])
+            return nfail(m, MS_ERR_ARG, fmt("%s: edges must be sorted by destination", who));
+        in_off[d + 1]++;
+    }
+    for (int n = 0; n < N; ++n) in_off[n + 1] += in_off[n];
+    return MS_OK;
 }
 
 // grads of Y = X W + b given dY: dW += X^T dY, db += colsum(dY), dX = dY W^T (if dX)
@@ -269,7 +297,10 @@ extern "C" void ms_nn_destroy(ms_nn* m) {
                     &m->dtmp_n2,
                     &m->dtmp_n3, &m->dout, &m->tgt, &m->terms})
         b->release();
-    for (auto* b : {&m->src, &m->dst, &m->in_off, &m->out_off, &m->out_perm, &m->goff, &m->gid}) b->release();
+    for (auto* b : {&m->src, &m->dst, &m->in_off, &m->out_off, &m->out_perm, &m->goff, &m->gid, &m->p_src, &m->p_dst,
+                    &m->p_inoff})
+        b->release();
+    for (auto* b : {&m->p_xn, &m->p_xe, &m->p_h, &m->p_e, &m->p_m, &m->p_out}) b->release();
     for (auto* v : {&m->h, &m->he, &m->cat_e, &m->z1, &m->a1, &m->m, &m->agg, &m->cat_n, &m->y1, &m->s1})
         for (auto& b : *v) b.release();
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -297,32 +328,17 @@ extern "C" int ms_nn_set_params(ms_nn* m, const float* in) {
 }
 
 extern "C" int ms_nn_set_graph(ms_nn* m, const ms_nn_graph_desc* g) {
-    if (!m || !g || g->num_graphs <= 0 || g->num_nodes <= 0 || g->num_edges <= 0 || !g->node_offsets ||
-        !g->edge_src || !g->edge_dst || !g->node_feat || !g->edge_feat)
-        return nfail(m, MS_ERR_ARG, "ms_nn_set_graph: bad arguments");
+    if (!m) return nfail(nullptr, MS_ERR_ARG, "ms_nn_set_graph: null network");
+    std::vector<int> in_off;
+    if (int r = check_graph(m, g, "ms_nn_set_graph", in_off)) return r;
     const int G = g->num_graphs, N = g->num_nodes, E = g->num_edges;
-    if (g->node_offsets[0] != 0 || g->node_offsets[G] != N)
-        return nfail(m, MS_ERR_ARG, "ms_nn_set_graph: node offsets must run from 0 to num_nodes");
-    std::vector<int> in_off(N + 1, 0), out_off(N + 1, 0), gid(N);
-    for (int e = 0; e < E; ++e) {
-        const int s = g->edge_src[e], d = g->edge_dst[e];
-        if (s < 0 || s >= N || d < 0 || d >= N) return nfail(m, MS_ERR_ARG, "ms_nn_set_graph: edge node out of range");
-        if (e > 0 && d < g->edge_dst[e - 1])
-            return nfail(m, MS_ERR_ARG, "ms_nn_set_graph: edges must be sorted by destination");
-        in_off[d + 1]++;
-        out_off[s + 1]++;
-    }
-    for (int n = 0; n < N; ++n) {
-        in_off[n + 1] += in_off[n];
-        out_off[n + 1] += out_off[n];
-    }
+    std::vector<int> out_off(N + 1, 0), gid(N);
+    for (int e = 0; e < E; ++e) out_off[g->edge_src[e] + 1]++;
+    for (int n = 0; n < N; ++n) out_off[n + 1] += out_off[n];
     std::vector<int> perm(E), cur(out_off.begin(), out_off.end() - 1);
     for (int e = 0; e < E; ++e) perm[cur[g->edge_src[e]]++] = e;  // ascending edge id per source
-    for (int q = 0; q < G; ++q) {
-        if (g->node_offsets[q + 1] < g->node_offsets[q])
-            return nfail(m, MS_ERR_ARG, "ms_nn_set_graph: node offsets must not decrease");
+    for (int q = 0; q < G; ++q)
         for (int n = g->node_offsets[q]; n < g->node_offsets[q + 1]; ++n) gid[n] = q;
-    }
     NNC(m, hipStreamSynchronize(m->stream));
     m->G = G;
     m->N = N;
@@ -382,6 +398,67 @@ extern "C" int ms_nn_forward(ms_nn* m, float* out) {
         NNC(m, hipStreamSynchronize(s));
         NNC(m, hipMemcpy(out, m->out.p, sizeof(float) * (size_t)N * m->arch.out_dim, hipMemcpyDeviceToHost));
     }
+    return MS_OK;
+}
+
+extern "C" int ms_nn_predict(ms_nn* m, const ms_nn_graph_desc* g, float* out) {
+    if (!m || !out) return nfail(m, MS_ERR_ARG, "ms_nn_predict: bad arguments");
+    const int W = m->arch.width, L = m->arch.layers;
+    if (W != 32 && W != 64) return nfail(m, MS_ERR_ARG, "ms_nn_predict: the fused kernels take width 32 or 64");
+    std::vector<int> in_off;
+    if (int r = check_graph(m, g, "ms_nn_predict", in_off)) return r;
+    const int N = g->num_nodes, E = g->num_edges, Fn = m->arch.node_features, Fe = m->arch.edge_features;
+    hipStream_t s = m->stream;
+    if (!m->num_cus) {
+        int dev = 0;
+        NNC(m, hipGetDevice(&dev));
+        NNC(m, hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    NNC(m, hipStreamSynchronize(s));
+    NNC(m, m->p_src.ensure(E));
+    NNC(m, m->p_dst.ensure(E));
+    NNC(m, m->p_inoff.ensure(N + 1));
+    NNC(m, m->p_xn.ensure((size_t)N * Fn));
+    NNC(m, m->p_xe.ensure((size_t)E * Fe));
+    NNC(m, m->p_h.ensure((size_t)N * W));
+    NNC(m, m->p_e.ensure((size_t)E * W));
+    NNC(m, m->p_m.ensure((size_t)std::max(N, E) * W));  // messages, then the decoder's hidden layer
+    NNC(m, m->p_out.ensure((size_t)N * m->arch.out_dim));
+    NNC(m, hipMemcpy(m->p_src.p, g->edge_src, E * sizeof(int), hipMemcpyHostToDevice));
+    NNC(m, hipMemcpy(m->p_dst.p, g->edge_dst, E * sizeof(int), hipMemcpyHostToDevice));
+    NNC(m, hipMemcpy(m->p_inoff.p, in_off.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+    NNC(m, hipMemcpy(m->p_xn.p, g->node_feat, (size_t)N * Fn * sizeof(float), hipMemcpyHostToDevice));
+    NNC(m, hipMemcpy(m->p_xe.p, g->edge_feat, (size_t)E * Fe * sizeof(float), hipMemcpyHostToDevice));
+    linear_fwd(m, m->enc_n, m->p_xn.p, N, m->p_h.p, true);
+    linear_fwd(m, m->enc_e, m->p_xe.p, E, m->p_e.p, true);
+    const float* P = m->params.p;
+    for (int l = 0; l < L; ++l) {
+        nn::MlpLayerArgs a{};
+        a.h = m->p_h.p;
+        a.e = m->p_e.p;
+        a.m = m->p_m.p;
+        a.src = m->p_src.p;
+        a.dst = m->p_dst.p;
+        a.in_off = m->p_inoff.p;
+        a.grid = 2 * m->num_cus;
+        a.W1 = P + m->e1[l].w;
+        a.b1 = P + m->e1[l].b;
+        a.W2 = P + m->e2[l].w;
+        a.b2 = P + m->e2[l].b;
+        a.M = E;
+        NNC(m, nn::mlp_layer(s, W, true, a));
+        a.W1 = P + m->n1[l].w;
+        a.b1 = P + m->n1[l].b;
+        a.W2 = P + m->n2[l].w;
+        a.b2 = P + m->n2[l].b;
+        a.M = N;
+        NNC(m, nn::mlp_layer(s, W, false, a));
+    }
+    linear_fwd(m, m->dec1, m->p_h.p, N, m->p_m.p, true);
+    linear_fwd(m, m->dec2, m->p_m.p, N, m->p_out.p);
+    NNC(m, hipGetLastError());
+    NNC(m, hipStreamSynchronize(s));
+    NNC(m, hipMemcpy(out, m->p_out.p, sizeof(float) * (size_t)N * m->arch.out_dim, hipMemcpyDeviceToHost));
     return MS_OK;
 }
 

@@ -29,7 +29,7 @@ EXPORTED = [
     "ms_steps_taken", "ms_resume_shell", "ms_set_tuning", "ms_get_tuning", "ms_enable_timing",
     "ms_last_step_timing", "ms_device_bytes", "ms_synchronize", "ms_calibration_copy", "ms_calibration_read",
     "ms_device_info", "ms_relax_setup", "ms_relax_run", "ms_set_check_log", "ms_read_check_log", "ms_nn_create", "ms_nn_destroy", "ms_nn_last_error", "ms_nn_num_params", "ms_nn_get_params",
-    "ms_nn_set_params", "ms_nn_set_graph", "ms_nn_forward", "ms_nn_loss_backward", "ms_nn_adam_step",
+    "ms_nn_set_params", "ms_nn_set_graph", "ms_nn_forward", "ms_nn_predict", "ms_nn_loss_backward", "ms_nn_adam_step",
     "ms_nn_gemm_test",
 ]
 
@@ -220,6 +220,7 @@ def lib() -> C.CDLL:
         "ms_nn_set_params": (C.c_int, [P, _fp]),
         "ms_nn_set_graph": (C.c_int, [P, C.POINTER(NNGraphDesc)]),
         "ms_nn_forward": (C.c_int, [P, _fp]),
+        "ms_nn_predict": (C.c_int, [P, C.POINTER(NNGraphDesc), _fp]),
         "ms_nn_loss_backward": (C.c_int, [P, _fp, _fp, _fp]),
         "ms_nn_adam_step": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
         "ms_nn_gemm_test": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, _fp]),

@@ -190,8 +190,8 @@ class WarmStartNet:
         net.set_params(z["params"])
         return net
 
-    def set_graph(self, b: dict) -> None:
-        self._graph = b  # keep the host arrays alive for the call
+    @staticmethod
+    def _desc(b: dict):
         d = nat.NNGraphDesc()
         d.num_graphs = len(b["offsets"]) - 1
         d.num_nodes = int(b["offsets"][-1])
@@ -201,7 +201,19 @@ class WarmStartNet:
         d.edge_dst = nat.iptr(b["dst"])
         d.node_feat = nat.fptr(b["node_feat"])
         d.edge_feat = nat.fptr(b["edge_feat"])
-        self._ck(self.L.ms_nn_set_graph(self.h, C.byref(d)))
+        return d
+
+    def set_graph(self, b: dict) -> None:
+        self._graph = b  # keep the host arrays alive for the call
+        self._ck(self.L.ms_nn_set_graph(self.h, C.byref(self._desc(b))))
+
+    def predict(self, b: dict) -> np.ndarray:
+        """The network's output for the batched graph b without the training
+        state (ms_nn_predict: fused kernels, no activation buffers); equal
+        bit for bit to set_graph(b) + forward()."""
+        out = np.empty((int(b["offsets"][-1]), 3), np.float32)
+        self._ck(self.L.ms_nn_predict(self.h, C.byref(self._desc(b)), nat.fptr(out)))
+        return out
 
     def forward(self) -> np.ndarray:
         n = int(self._graph["offsets"][-1])
@@ -224,6 +236,8 @@ class WarmStartNet:
 
     def predict_dofs(self, sh, graph: dict | None = None) -> np.ndarray:
         g = graph or shell_graph(sh)
+        if self.arch[0] in (32, 64):
+            return positions_from_output(sh, g, self.predict(batch_graphs([g])))
         self.set_graph(batch_graphs([g]))
         return positions_from_output(sh, g, self.forward())
 

@@ -53,6 +53,43 @@ def _small_batch():
 
 
 @pytest.mark.parametrize("width,layers", [(32, 2), (64, 4)])
+def test_fused_inference_equals_training_forward(width, layers):
+    """ms_nn_predict (fused half-layers, no activation buffers) gives the
+    training forward's output bit for bit: the same bfloat16 operand
+    roundings, MFMA sequence, in-edge summation order and FP32 epilogues.  A
+    batch of three 9k-node discs (about 170k edges: thousands of 64-row
+    tiles over the persistent blocks) and the small two-graph batch; the
+    training graph set before stays usable after a predict call."""
+    from morphoshell_amd.meshing import disc_mesh, grf, isotropic_abar_info, uniform_bbar_info
+    from morphoshell_amd.shell import Settings, prepare_shell
+    from morphoshell_amd.warmstart import WarmStartNet, batch_graphs, shell_graph
+
+    shells = []
+    for i in range(3):
+        m = disc_mesh(0.02, seed=i)
+        m.abar_info = isotropic_abar_info(1.0 + 0.1 * grf(m.centroids()[:, :2], 0.3, seed=i))
+        m.bbar_info = uniform_bbar_info(m.num_tris, 0.3 * i)
+        m.fill_defaults(h=0.03)
+        shells.append(prepare_shell(m, Settings()))
+    big = batch_graphs([shell_graph(s) for s in shells])
+    _, _, small = _small_batch()
+    net = WarmStartNet(seed=11, width=width, layers=layers)
+    try:
+        for b in (small, big):
+            net.set_graph(b)
+            ref = net.forward()
+            got = net.predict(b)
+            print(f"nodes {ref.shape[0]}, edges {len(b['src'])}: max |out| {np.abs(ref).max():.3e}")
+            assert np.array_equal(got, ref)
+        net.set_graph(small)
+        again = net.forward()
+        net.predict(big)
+        assert np.array_equal(net.forward(), again)
+    finally:
+        net.close()
+
+
+@pytest.mark.parametrize("width,layers", [(32, 2), (64, 4)])
 def test_forward_and_gradients_match_reference(width, layers):
     """At the configured architecture (4 layers, width 64) and a smaller one."""
     from morphoshell_amd.warmstart import EDGE_FEATURES, NODE_FEATURES, WarmStartNet
