@@ -26,6 +26,8 @@ Other workloads (one JSON line each):
   --workload c5           the thin-sheet stress test (warm start, then mixed), bounded
   --workload batch_relax  a 512-shell batch of 500-4000 triangles relaxed to
                           equilibrium, FP64 vs mixed-then-FP64
+  --workload nn           the warm-start network: fused inference on the large
+                          shell's graph, and training steps on a batch
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -724,6 +726,82 @@ def _relax_series(sh, precision, x, v, K, steps, chunk, from_ansatz=False):
     return pts, xe, ve, E, bool(r["failed"])
 
 
+def nn_counts(W: int, L: int, N: int, E: int, Fn: int, Fe: int) -> dict:
+    """Algorithmic flops and bytes of one inference (ms_nn_predict) of the
+    network (width W, L layers) on a graph of N nodes and E directed edges.
+    Per layer, edge half: 2 E (3W W + W W) flops, reading h once (4W B per
+    node), e and both node ids, writing m and e; node half: 2 N (2W W + W W)
+    flops, reading h and m, writing h.  Encoders and decoder: their GEMMs."""
+    edge_f, node_f = 8 * E * W * W, 6 * N * W * W
+    edge_b = N * 4 * W + E * (12 * W + 8)
+    node_b = N * (8 * W + 4) + E * 4 * W
+    enc_f = 2 * (N * Fn + E * Fe) * W + 2 * N * (W * W + 3 * W)
+    enc_b = 4 * (N * Fn + E * Fe) + 4 * (N + E) * W + 4 * N * (W + W + 3)
+    return {"edge_half": {"flops": edge_f, "bytes": edge_b}, "node_half": {"flops": node_f, "bytes": node_b},
+            "total_flops": L * (edge_f + node_f) + enc_f, "total_bytes": L * (edge_b + node_b) + enc_b}
+
+
+def bench_nn(args) -> dict:
+    """The learned warm start's network on the GPU: ms_nn_predict (fused
+    half-layers) on the graph of the large single shell, and training steps
+    (forward, loss, backward, Adam) on a batch of training-family graphs."""
+    from morphoshell_amd import warmstart as ws
+    from morphoshell_amd import workloads
+
+    sh = workloads.grf_disc(args.spacing)
+    t0 = time.perf_counter()
+    g = ws.shell_graph(sh)
+    b = ws.batch_graphs([g])
+    t_graph = time.perf_counter() - t0
+    net = ws.WarmStartNet.load(ws.PRETRAINED_DEFECT)
+    W, L = net.arch
+    N, E = int(b["offsets"][-1]), len(b["src"])
+    counts = nn_counts(W, L, N, E, b["node_feat"].shape[1], b["edge_feat"].shape[1])
+    for _ in range(2):
+        net.predict(b)
+    reps = max(1, args.nn_reps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = net.predict(b)
+    t_pred = (time.perf_counter() - t0) / reps
+    log(f"predict on N={N} E={E}: {t_pred * 1e3:.1f} ms per call (graph upload and output copy included)")
+    # training steps on a batch of 8 training-family graphs
+    shells = ws.training_family(8, 424_242, family="defect")
+    graphs = [ws.shell_graph(s) for s in shells]
+    tb = ws.batch_graphs(graphs)
+    tgt = np.zeros((int(tb["offsets"][-1]), 3), np.float32)
+    net.set_graph(tb)
+    for _ in range(3):
+        net.train_step(tgt, 1e-5)
+    t0 = time.perf_counter()
+    for _ in range(reps * 4):
+        net.train_step(tgt, 1e-5)
+    t_train = (time.perf_counter() - t0) / (reps * 4)
+    net.close()
+    cal = None
+    if args.calibrate:
+        from morphoshell_amd import native
+
+        cal_ms = np.zeros(1)
+        native.check(native.lib().ms_calibration_copy(1 << 30, 5, native.dptr(cal_ms)))
+        cal = {"copy_kernel_GBs": 2 * (1 << 30) / (cal_ms[0] * 1e-3) / 1e9}
+    return {
+        "calibration": cal,
+        "metric": "warm-start network inference (ms_nn_predict) on the large shell's graph",
+        "value": t_pred * 1e3,
+        "unit": "ms per prediction (graph upload and output copy included)",
+        "higher_is_better": False,
+        "arch": {"width": W, "layers": L},
+        "graph": {"nodes": N, "edges": E, "graph_build_s": t_graph, "shell_triangles": sh.T},
+        "algorithmic": counts,
+        "wall_TFLOPs": counts["total_flops"] / t_pred / 1e12,
+        "output_max_abs": float(np.abs(out).max()),
+        "train_step_ms": t_train * 1e3,
+        "train_batch": {"graphs": len(graphs), "nodes": int(tb["offsets"][-1]), "edges": len(tb["src"])},
+        "kernel_profile": "profiles/nn_mlp_layer_grf_disc_1.8M.txt",
+    }
+
+
 def bench_c5(args) -> dict:
     """The thin-sheet stress test with the chosen parts together, bounded:
     a disc at spacing 0.001 (7.25M triangles), h = 0.002, a +1 defect metric
@@ -1001,7 +1079,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax"])
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax", "nn"])
+    ap.add_argument("--nn-reps", type=int, default=5, help="nn workload: timed predictions")
     ap.add_argument("--spacing", type=float, default=0.002)
     ap.add_argument("--shells", type=int, default=512)
     ap.add_argument("--relax-steps", type=int, default=3_000_000,
@@ -1035,7 +1115,8 @@ def main() -> int:
               "runs on several GPUs are not part of it.", file=sys.stderr)
         return 2
     fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax,
-          "relax_large": bench_relax_large, "c5": bench_c5, "batch_relax": bench_batch_relax}[args.workload]
+          "relax_large": bench_relax_large, "c5": bench_c5, "batch_relax": bench_batch_relax,
+          "nn": bench_nn}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -138,9 +138,63 @@ __device__ __forceinline__ void wave_gemm(const short* A, int lda, const short* 
     }
 }
 
+// A thread's share of a tile: the gathered A-tile values (K1 / 16 float4 of
+// the 64 x K1 concatenated rows) and the residual operand of its epilogue
+// elements (e in the edge half, h in the node half), all loaded at the start
+// of the tile, so that the epilogue does not wait on a load.
+template <int W, bool EDGE>
+struct MlpStage {
+    static constexpr int K1 = EDGE ? 3 * W : 2 * W, Q = K1 / 4, NQ = 64 * Q / 256;
+    float4 v[NQ];
+    float res[W / 16][4];
+};
+
+template <int W, bool EDGE>
+__device__ __forceinline__ void mlp_gather(const MlpLayerArgs& a, int r0, MlpStage<W, EDGE>& st) {
+    typedef MlpStage<W, EDGE> S;
+    constexpr int QW = W / 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int j = 0; j < S::NQ; ++j) {
+        const int i = tid + j * 256;
+        const int r = i / S::Q, q = i % S::Q, row = r0 + r;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < a.M) {
+            const int seg = q / QW, c = (q % QW) * 4;
+            if (EDGE) {
+                const float* p = seg == 0   ? a.h + (size_t)a.dst[row] * W
+                                 : seg == 1 ? a.h + (size_t)a.src[row] * W
+                                            : a.e + (size_t)row * W;
+                v = *reinterpret_cast<const float4*>(p + c);
+            } else if (seg == 0) {
+                v = *reinterpret_cast<const float4*>(a.h + (size_t)row * W + c);
+            } else {
+                // the node's in-edge sum, edge by edge in CSR order
+                for (int e = a.in_off[row]; e < a.in_off[row + 1]; ++e) {
+                    const float4 x = *reinterpret_cast<const float4*>(a.m + (size_t)e * W + c);
+                    v.x += x.x;
+                    v.y += x.y;
+                    v.z += x.z;
+                    v.w += x.w;
+                }
+            }
+        }
+        st.v[j] = v;
+    }
+    const float* R = EDGE ? a.e : a.h;
+#pragma unroll
+    for (int t = 0; t < W / 16; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = r0 + wave * 16 + (lane >> 4) * 4 + r;
+            st.res[t][r] = row < a.M ? R[(size_t)row * W + t * 16 + (lane & 15)] : 0.f;
+        }
+}
+
 template <int W, bool EDGE>
 __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
-    constexpr int K1 = EDGE ? 3 * W : 2 * W;
+    typedef MlpStage<W, EDGE> S;
+    constexpr int K1 = S::K1;
     constexpr int L1 = K1 + 8, L2 = W + 8;
     extern __shared__ __attribute__((aligned(16))) short smem_mlp[];
     short* W1t = smem_mlp;
@@ -148,42 +202,22 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
     short* As = W2t + W * L2;
     short* Ts = As + 64 * L1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntiles = (a.M + 63) / 64;
+    if ((int)blockIdx.x >= ntiles) return;  // block-uniform
+    S cur;
     // weights, once per block: read k-major (coalesced over the output index), stored transposed
     for (int i = tid; i < K1 * W; i += 256) W1t[(i % W) * L1 + i / W] = f2bf(a.W1[i]);
     for (int i = tid; i < W * W; i += 256) W2t[(i % W) * L2 + i / W] = f2bf(a.W2[i]);
-    const int ntiles = (a.M + 63) / 64;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int r0 = tile * 64;
-        // A tile: 64 rows of K1 values, four at a time
-        constexpr int Q = K1 / 4, QW = W / 4;
-        for (int i = tid; i < 64 * Q; i += 256) {
-            const int r = i / Q, q = i % Q, row = r0 + r;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (row < a.M) {
-                const int seg = q / QW, c = (q % QW) * 4;
-                if (EDGE) {
-                    const float* p = seg == 0   ? a.h + (size_t)a.dst[row] * W
-                                     : seg == 1 ? a.h + (size_t)a.src[row] * W
-                                                : a.e + (size_t)row * W;
-                    v = *reinterpret_cast<const float4*>(p + c);
-                } else if (seg == 0) {
-                    v = *reinterpret_cast<const float4*>(a.h + (size_t)row * W + c);
-                } else {
-                    // the node's in-edge sum, edge by edge in CSR order
-                    for (int e = a.in_off[row]; e < a.in_off[row + 1]; ++e) {
-                        const float4 x = *reinterpret_cast<const float4*>(a.m + (size_t)e * W + c);
-                        v.x += x.x;
-                        v.y += x.y;
-                        v.z += x.z;
-                        v.w += x.w;
-                    }
-                }
-            }
-            short* d = As + r * L1 + q * 4;
-            d[0] = f2bf(v.x);
-            d[1] = f2bf(v.y);
-            d[2] = f2bf(v.z);
-            d[3] = f2bf(v.w);
+        mlp_gather<W, EDGE>(a, r0, cur);
+        // this tile's A rows into LDS as bfloat16, four values per 8-byte store
+#pragma unroll
+        for (int j = 0; j < S::NQ; ++j) {
+            const int i = tid + j * 256;
+            const int r = i / S::Q, q = i % S::Q;
+            const float4 v = cur.v[j];
+            *reinterpret_cast<v4s*>(As + r * L1 + q * 4) = v4s{f2bf(v.x), f2bf(v.y), f2bf(v.z), f2bf(v.w)};
         }
         __syncthreads();
         v4f acc[W / 16];
@@ -213,11 +247,9 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
                 const float v = acc[t][r] + b;
                 if (EDGE) {
                     a.m[(size_t)row * W + col] = v;
-                    float* ep = a.e + (size_t)row * W + col;
-                    *ep = *ep + v;
+                    a.e[(size_t)row * W + col] = cur.res[t][r] + v;
                 } else {
-                    float* hp = a.h + (size_t)row * W + col;
-                    *hp = *hp + v;
+                    a.h[(size_t)row * W + col] = cur.res[t][r] + v;
                 }
             }
         }
