@@ -506,9 +506,11 @@ def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: i
     }
 
 
-# Parameters trained on 2655 equilibria of the defect family (1024 shells from
-# seed 1 and 2048 from seed 200000, as generate() makes them; 40 000 Adam steps
-# at lr 2e-3; loss 0.274 -> 0.196), written by WarmStartNet.save.
+# Parameters trained on 5789 equilibria of the defect family (2048 shells
+# from each of the family seeds 1, 200000 and 300000, as generate() makes them
+# in batches of 256, 600 000 steps at most, tolerance 1e-5; 60 000 Adam steps
+# of 8 graphs at lr 2e-3 with cosine decay; loss 0.251 -> 0.189), written by
+# WarmStartNet.save.
 PRETRAINED_DEFECT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_defect.npz")
 
 

@@ -200,19 +200,20 @@ def test_warm_start_dialled_from_its_own_geometry_starts_force_free():
 
 def test_pretrained_network_on_unseen_shells_saves_steps():
     """The shipped parameters (4 layers, width 64, trained on equilibria of the
-    defect family the batched solver generated) applied to 24 shells of that
+    defect family the batched solver generated) applied to 64 shells of that
     family from seeds outside the training set: the energy test keeps most
     predictions, every kept start has a lower elastic energy at dial factor 1
-    than the flat start with noise, and over the shells both starts bring to
-    equilibrium (residual force and speed 1e-5 of their scales) the warm
-    start takes fewer steps than the flat start in the median (steps ratio
-    flat / warm above 1).  The runs are deterministic, so the ratio is a
+    than the flat start with noise, at least as many warm starts as flat ones
+    reach equilibrium (residual force and speed 1e-5 of their scales), and
+    over the shells both bring there the warm start takes fewer steps than
+    the flat start in the median (steps ratio flat / warm above 1.1; 1.25 on
+    256 other unseen shells).  The runs are deterministic, so the ratio is a
     fixed number of this code and these seeds; it is printed."""
     from morphoshell_amd import warmstart as ws
 
     net = ws.WarmStartNet.load(ws.PRETRAINED_DEFECT)
     assert net.arch == (64, 4)
-    shells = ws.training_family(24, 987_000, family="defect")
+    shells = ws.training_family(64, 987_000, family="defect")
     plans, used = ws.warm_start_plans(net, shells)
     n_used = sum(u is not None for u in used)
     from morphoshell_amd.workloads import flat_noisy_state
@@ -226,9 +227,13 @@ def test_pretrained_network_on_unseen_shells_saves_steps():
     warm, _, tw = ws.relax_batch(shells, 1e-5, 600_000, plans=plans)
     both = [i for i in range(len(shells)) if base[i]["status"] == 2 and warm[i]["status"] == 2]
     ratio = np.array([(base[i]["eq_step"] + 1) / (warm[i]["eq_step"] + 1) for i in both])
-    print(f"predictions kept {n_used}/24; both converged {len(both)}; steps ratio flat/warm median "
-          f"{np.median(ratio):.3f} (min {ratio.min():.2f}, max {ratio.max():.2f}); {tb:.1f}s flat, {tw:.1f}s warm")
+    n_base = sum(r["status"] == 2 for r in base)
+    n_warm = sum(r["status"] == 2 for r in warm)
+    print(f"predictions kept {n_used}/64; converged flat {n_base}, warm {n_warm}, both {len(both)}; steps ratio "
+          f"flat/warm median {np.median(ratio):.3f} (min {ratio.min():.2f}, max {ratio.max():.2f}); "
+          f"{tb:.1f}s flat, {tw:.1f}s warm")
     net.close()
-    assert n_used >= 18
-    assert len(both) >= 18
-    assert np.median(ratio) > 1.0
+    assert n_used >= 48
+    assert n_warm >= n_base
+    assert len(both) >= 48
+    assert np.median(ratio) > 1.1
