@@ -28,6 +28,9 @@ Other workloads (one JSON line each):
                           equilibrium, FP64 vs mixed-then-FP64
   --workload nn           the warm-start network: fused inference on the large
                           shell's graph, and training steps on a batch
+  --workload relax_large_branch
+                          the mixed relax_large run replayed to a state before
+                          its blow-up, then continued in FP64 and in mixed
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -690,6 +693,79 @@ def bench_relax_large(args) -> dict:
     }
 
 
+def bench_relax_large_branch(args) -> dict:
+    """Why the large shell's mixed relaxation blows up.  The mixed run of
+    --workload relax_large is replayed (bit for bit: the device results do not
+    depend on chunking or launch shape) to the first chunk at or after
+    --branch-at steps; from that state it continues once in FP64 and once in
+    mixed precision (--branch-steps at most, until a blow-up), recording the
+    equilibrium-check residual, the largest speed and, in the mixed branch,
+    the elastic forces of the current state in both precisions (the mixed
+    force error; computing them does not touch the state)."""
+    from morphoshell_amd import workloads
+    from morphoshell_amd.shell import Context
+
+    sh = workloads.grf_disc(args.spacing)
+    x0, v0 = workloads.flat_noisy_state(sh)
+    K, interval, chunk = 20000, 1000, 4000
+    fchar = sh.scales["char_force_scale"]
+    out = {"workload": f"single disc shell, spacing {args.spacing}, {sh.T} triangles, mixed relaxation dialled over "
+                       f"{K} steps, checks every {interval}"}
+    with Context(sh, precision="mixed") as ctx:
+        ctx.set_tuning(**LARGE_FUSED["mixed"])
+
+        def setup():
+            ctx.set_state(0, x0, v0)
+            ctx.relax_setup(0, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=0.0, equil_interval=interval,
+                            tol_f=1e-6, tol_v=1e-5)
+
+        setup()
+        ctx.relax_run(chunk, poll_every=chunk)  # warm: graphs captured, tiles built
+        setup()
+        ctx.set_check_log(1 << 16)
+        while True:
+            r = ctx.relax_run(chunk, poll_every=chunk)[0]
+            if r["failed"] or r["steps"] >= args.branch_at:
+                break
+        recs, _ = ctx.read_check_log(0)
+        res = np.array([c["max_force_ratio"] for c in recs if c["kind"] == 0 and c["step"] > K])
+        out["replay"] = {"steps": int(r["steps"]), "failed": int(r["failed"]),
+                         "residual_after_dial": {"median": float(np.median(res)), "max": float(res.max()),
+                                                 "last": float(res[-1])} if res.size else None}
+        log(f"replayed the mixed run to step {r['steps']}: {out['replay']}")
+        if r["failed"]:
+            return out
+        snap = ctx.get_state(0)
+        for name, prec in (("fp64", "fp64"), ("mixed", "mixed")):
+            ctx.set_precision(prec)
+            ctx.set_state(0, *snap)
+            ctx.resume(0)
+            start = ctx.relax_run(0, poll_every=500)[0]["steps"]
+            rows = []
+            while True:
+                r = ctx.relax_run(500, poll_every=500)[0]
+                x, v = ctx.get_state(0)
+                row = {"steps": int(r["steps"] - start), "failed": int(r["failed"]),
+                       "check_residual": float(r["max_force_ratio"]), "max_speed": float(np.abs(v).max())}
+                if prec == "mixed" and not r["failed"]:
+                    ctx.set_precision("fp64")
+                    ctx.calc_deformation_forces()
+                    f64 = ctx.get_forces(0)
+                    ctx.set_precision("mixed")
+                    ctx.calc_deformation_forces()
+                    fm = ctx.get_forces(0)
+                    a = float(np.abs(f64).max())
+                    row.update(elastic_max_over_fchar=a / fchar,
+                               mixed_err_rel=float(np.abs(fm - f64).max()) / a,
+                               mixed_err_over_fchar=float(np.abs(fm - f64).max()) / fchar)
+                rows.append(row)
+                if r["failed"] or r["steps"] - start >= args.branch_steps:
+                    break
+            out[name] = {"blew_up_after": rows[-1]["steps"] if rows[-1]["failed"] else None, "rows": rows}
+            log(f"{name} branch: {rows[-1]}")
+    return out
+
+
 def _relax_series(sh, precision, x, v, K, steps, chunk, from_ansatz=False):
     """Relax one shell under the device-run protocol (dialled over K steps,
     from its start geometry when from_ansatz) for a bounded number of steps;
@@ -1080,7 +1156,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="single",
-                    choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax", "nn"])
+                    choices=["single", "batch", "relax", "relax_large", "c5", "batch_relax", "nn",
+                             "relax_large_branch"])
+    ap.add_argument("--branch-at", type=int, default=390000, help="relax_large_branch: replay the mixed run this far")
+    ap.add_argument("--branch-steps", type=int, default=12000, help="relax_large_branch: steps of each branch")
     ap.add_argument("--nn-reps", type=int, default=5, help="nn workload: timed predictions")
     ap.add_argument("--spacing", type=float, default=0.002)
     ap.add_argument("--shells", type=int, default=512)
@@ -1116,7 +1195,7 @@ def main() -> int:
         return 2
     fn = {"single": bench_single, "batch": bench_batch, "relax": bench_relax,
           "relax_large": bench_relax_large, "c5": bench_c5, "batch_relax": bench_batch_relax,
-          "nn": bench_nn}[args.workload]
+          "nn": bench_nn, "relax_large_branch": bench_relax_large_branch}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 
