@@ -125,9 +125,15 @@ BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_step_ragged_batch_5
 # The fused step's fastest shapes on the 1.8M-triangle disc, both on 512-node
 # tiles and a persistent grid of two workgroups per CU: FP64 as two chunks of
 # 256 nodes (256-thread workgroups, passes of 256 triangles, so two
-# workgroups fit a CU's LDS), mixed as one chunk of 512.
-LARGE_FUSED = {"fp64": dict(node_block=256, tile_chunks=2, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2),
-               "mixed": dict(node_block=512, tile_chunks=1, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2)}
+# workgroups fit a CU's LDS), mixed as one chunk of 512.  Tiles balanced
+# over the grid's workgroups (tile_nodes=-2: 2046 tiles of 444 nodes, 4 per
+# workgroup, against 1774 of 512 in 3.46 rounds) measured slower, 0.152
+# against 0.144 ms in FP64 and 0.108 against 0.098 mixed: the last round's
+# few workgroups run faster, and smaller tiles recompute more halo.
+LARGE_FUSED = {"fp64": dict(node_block=256, tile_chunks=2, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2,
+                            tile_nodes=-1),
+               "mixed": dict(node_block=512, tile_chunks=1, fused=1, tri_grid_per_cu=0, node_grid_per_cu=2,
+                             tile_nodes=-1)}
 
 
 def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
@@ -271,7 +277,10 @@ def bench_single(args) -> dict:
     for prec in args.precisions.split(","):
         with Context(sh, precision=prec) as ctx:
             for form in args.forms.split(","):
-                ctx.set_tuning(**(LARGE_FUSED[prec] if form == "fused" else two_launch))
+                ctx.set_tuning(**(dict(LARGE_FUSED[prec], tile_nodes=args.tile_nodes) if form == "fused"
+                                  else two_launch))
+                if form == "fused":
+                    log(f"{prec} fused tiles of {ctx.get_tuning()['tile_nodes']} nodes")
                 secs = time_steps(ctx, states, args.steps, args.warmup)
                 kt = kernel_times(ctx, states, min(args.steps, 50))
                 res[(prec, form)] = {"secs": secs, "kt": kt}
@@ -1185,6 +1194,8 @@ def main() -> int:
     ap.add_argument("--no-calibrate", action="store_true",
                     help="single workload: skip the copy and read kernels that measure the HBM ceiling")
     ap.add_argument("--precisions", default="fp64,mixed", help="single workload: precisions to time")
+    ap.add_argument("--tile-nodes", type=int, default=-1,
+                    help="single workload: nodes per fused tile (-2 balanced over the grid, -1 full chunks)")
     ap.add_argument("--forms", default="fused,two_launch", help="single workload: step forms to time")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: the autotuner's usual pick, no search")

@@ -161,6 +161,8 @@ typedef struct ms_shell_desc {
     const int32_t* patch_ids;      /* 6T */
     const double* G;               /* 6T */
     const double* H;               /* 18T */
+    const double* ref_pos;         /* 3N dofs SoA: the reference positions G and H
+                                      were fitted on (x and y are read) */
     const double* ref_areas;       /* T */
     const double* thickness;       /* T  def_thicknesses */
     const double* shear_modulus;   /* T  def_shear_moduli */
@@ -387,7 +389,18 @@ typedef struct ms_tuning {
     int32_t tile_chunks;      /* node chunks per tile of the fused step: 1 or 2
                                  (larger tiles recompute fewer halo triangles);
                                  0 keeps */
-    int32_t reserved;         /* 0 */
+    int32_t tile_nodes;       /* nodes per tile of the fused step: each shell's
+                                 nodes are cut into groups of this many (a tile),
+                                 and each group into node_block chunks.
+                                 > 0: that size, at most tile_chunks * node_block;
+                                 -1: tile_chunks * node_block (the default);
+                                 -2: balanced: when full tiles are fewer than
+                                 the launch's workgroups, the smallest size (at
+                                 least 64) whose tiles still fit one round of
+                                 them, so that no workgroup idles; else full;
+                                 0 keeps.  States are bit-identical for every
+                                 choice.  ms_get_tuning reports the size in
+                                 force. */
 } ms_tuning;
 int ms_set_tuning(ms_ctx* ctx, const ms_tuning* t);
 int ms_get_tuning(const ms_ctx* ctx, ms_tuning* t);

@@ -29,22 +29,24 @@ DEFAULT_SPACE = {
     "graph_steps": (16,),
     "fused_node_block": (128, 256, 512),  # block sizes of the fused step
     "fused_tile_chunks": (1, 2),      # node chunks per tile of the fused step
+    "fused_tile_nodes": (-1, -2),     # full tiles, or cut to fill the grid (same when they already do)
 }
 
 
 def _candidates(sp: dict):
     for tb, nb, g, gs in itertools.product(sp["tri_block"], sp["node_block"], sp["grid_per_cu"], sp["graph_steps"]):
         yield {"tri_block": tb, "node_block": nb, "grid_per_cu": g, "graph_steps": gs, "fused": 0}
-    for nb, tc, g, gs in itertools.product(sp.get("fused_node_block", ()), sp.get("fused_tile_chunks", (1,)),
-                                           sp["grid_per_cu"], sp["graph_steps"]):
-        yield {"tri_block": 256, "node_block": nb, "tile_chunks": tc, "grid_per_cu": g, "graph_steps": gs,
-               "fused": 1}
+    for nb, tc, tn, g, gs in itertools.product(sp.get("fused_node_block", ()), sp.get("fused_tile_chunks", (1,)),
+                                               sp.get("fused_tile_nodes", (-1,)), sp["grid_per_cu"],
+                                               sp["graph_steps"]):
+        yield {"tri_block": 256, "node_block": nb, "tile_chunks": tc, "tile_nodes": tn, "grid_per_cu": g,
+               "graph_steps": gs, "fused": 1}
 
 
 def _apply(ctx: Context, c: dict) -> None:
     ctx.set_tuning(tri_block=c["tri_block"], node_block=c["node_block"], tri_grid_per_cu=c["grid_per_cu"],
                    node_grid_per_cu=c["grid_per_cu"], graph_steps=c["graph_steps"], fused=c.get("fused", 0),
-                   tile_chunks=c.get("tile_chunks", 1))
+                   tile_chunks=c.get("tile_chunks", 1), tile_nodes=c.get("tile_nodes", -1))
 
 
 def _cache_path() -> str:
@@ -80,7 +82,7 @@ def signature(ctx: Context) -> str:
     S = len(ctx.shells)
     b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
     prec = "mixed" if ctx.precision == 1 else "fp64"
-    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-{_device()}-v4"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-{_device()}-v5"
 
 
 def _load() -> dict:
@@ -135,6 +137,8 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
             if verbose:
                 print(f"[autotune] {sig} {cand}: refused ({e})", file=sys.stderr)
             continue
+        if cand.get("tile_nodes") == -2 and ctx.get_tuning()["tile_nodes"] == cand["tile_chunks"] * cand["node_block"]:
+            continue  # the tiles already fill the grid: the same shape as full tiles
         t = time_config(ctx, steps)
         results.append((t, cand))
         if verbose:

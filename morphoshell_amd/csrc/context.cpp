@@ -40,6 +40,7 @@ struct ms_ctx {
     double *h = nullptr, *mu = nullptr, *area = nullptr;
     float *G32 = nullptr, *H32 = nullptr, *hmuA32 = nullptr;
     double* mass = nullptr;
+    double* ref = nullptr;  // 2N reference x, y (SoA)
     unsigned char* clamp = nullptr;
     long long* csr_off = nullptr;
     int* csr_ent = nullptr;
@@ -56,6 +57,10 @@ struct ms_ctx {
     int n_tri_chunks = 0, n_node_chunks = 0;
     size_t part_len = 0;
     std::vector<Chunk> h_tri_chunks, h_node_chunks;  // host copies of the full tables
+    // tiles of the fused step: {first node, end node, shell, first chunk} of
+    // each group of tile_L nodes (cut into node chunks by build_chunks)
+    std::vector<int4> h_tile_groups;
+    int tile_L = 0;
 
     // Step launches over the running shells only (see refresh_active): the
     // running shells' chunks and tiles, rebuilt when the set of halted shells
@@ -158,9 +163,12 @@ hipError_t dalloc(ms_ctx* c, T** p, size_t n) {
 // Chunk tables for blocks of tb triangles and nb nodes, all-or-nothing: the
 // new tables and partial buffers are allocated and filled before the old
 // ones are freed, so a failed allocation leaves the context on its current
-// tables (and its current tiles and graph, which name them).
-int build_chunks(ms_ctx* c, int tb, int nb) {
+// tables (and its current tiles and graph, which name them).  Each shell's
+// nodes are cut into groups of L nodes (the tiles of the fused step) and each
+// group into chunks of nb, so chunk r of a tile starts r * nb nodes into it.
+int build_chunks(ms_ctx* c, int tb, int nb, int L) {
     std::vector<Chunk> tc, nc;
+    std::vector<int4> groups;
     std::vector<ShellDev> hs = c->hs;
     for (int s = 0; s < c->num_shells; ++s) {
         ShellDev& S = hs[s];
@@ -169,8 +177,12 @@ int build_chunks(ms_ctx* c, int tb, int nb) {
             tc.push_back(make_int4(t, std::min(t + tb, S.tri_begin + S.num_tris), s, (int)tc.size()));
         S.tri_chunk_count = (int)tc.size() - S.tri_chunk_begin;
         S.node_chunk_begin = (int)nc.size();
-        for (int n = S.node_begin; n < S.node_begin + S.num_nodes; n += nb)
-            nc.push_back(make_int4(n, std::min(n + nb, S.node_begin + S.num_nodes), s, (int)nc.size()));
+        const int nend = S.node_begin + S.num_nodes;
+        for (int g = S.node_begin; g < nend; g += L) {
+            const int gend = std::min(g + L, nend);
+            groups.push_back(make_int4(g, gend, s, (int)nc.size()));
+            for (int n = g; n < gend; n += nb) nc.push_back(make_int4(n, std::min(n + nb, gend), s, (int)nc.size()));
+        }
         S.node_chunk_count = (int)nc.size() - S.node_chunk_begin;
     }
     const size_t part_len = 4 * std::max(tc.size(), nc.size());
@@ -204,6 +216,8 @@ int build_chunks(ms_ctx* c, int tb, int nb) {
     c->n_node_chunks = (int)nc.size();
     c->h_tri_chunks = std::move(tc);
     c->h_node_chunks = std::move(nc);
+    c->h_tile_groups = std::move(groups);
+    c->tile_L = L;
     c->hs = std::move(hs);
     c->act_live.clear();
     c->act_all = true;
@@ -285,23 +299,14 @@ int build_tiles(ms_ctx* c) {
     const bool mixed = c->precision == MS_MIXED;
     std::vector<int> pid((size_t)6 * T), ent;
     std::vector<long long> off((size_t)N + 1);
-    std::vector<Chunk> nc(c->n_node_chunks);
     HIPC(c, hipMemcpy(pid.data(), c->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPC(c, hipMemcpy(off.data(), c->csr_off, off.size() * sizeof(long long), hipMemcpyDeviceToHost));
     ent.resize((size_t)off[N]);
     HIPC(c, hipMemcpy(ent.data(), c->csr_ent, ent.size() * sizeof(int), hipMemcpyDeviceToHost));
-    HIPC(c, hipMemcpy(nc.data(), c->d_node_chunks, nc.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
-    // tiles: groups of up to tile_chunks consecutive chunks of one shell
+    // tiles: the node groups the chunks were cut from (at most tile_chunks
+    // chunks each)
     const int npt = std::max(1, std::min(MAX_TILE_CHUNKS, c->tune.tile_chunks));
-    std::vector<int4> tiles;
-    for (int s = 0; s < c->num_shells; ++s) {
-        const ShellDev& S = c->hs[s];
-        for (int k = 0; k < S.node_chunk_count; k += npt) {
-            const int first = S.node_chunk_begin + k;
-            const int last = S.node_chunk_begin + std::min(k + npt, S.node_chunk_count) - 1;
-            tiles.push_back(make_int4(nc[first].x, nc[last].y, s, first));
-        }
-    }
+    const std::vector<int4>& tiles = c->h_tile_groups;
     // per pass npt * B + 1 offsets, padded to an even count so that every
     // tile's table starts on a 32-bit word (the kernel stages it by words)
     const int stride = npt * B + 2;
@@ -367,6 +372,13 @@ int build_tiles(ms_ctx* c) {
     for (size_t ci = 0; ci < tiles.size(); ++ci) {
         const int n0 = tiles[ci].x, n1 = tiles[ci].y, nn = n1 - n0;
         if (nn > npt * B) return fail(c, MS_ERR_ARG, "fused step: a tile is larger than its chunks");
+        // the kernel takes chunk r of a tile to start r * B nodes into it
+        for (int r = 0; r * B < nn; ++r) {
+            const int k = tiles[ci].w + r;
+            if (k >= c->n_node_chunks || c->h_node_chunks[k].x != n0 + r * B ||
+                c->h_node_chunks[k].y != std::min(n0 + (r + 1) * B, n1))
+                return fail(c, MS_ERR_ARG, "fused step: a tile's chunks do not match its nodes");
+        }
         const std::vector<int>& tl = tile_tris[ci];
         const int npass = (int)((tl.size() + split - 1) / split);
         hl.clear();
@@ -527,6 +539,7 @@ TriData tri_data(const ms_ctx* c) {
     td.mu = c->mu;
     td.area = c->area;
     td.hmuA32 = c->hmuA32;
+    td.ref = c->ref;
     return td;
 }
 
@@ -534,6 +547,7 @@ NodeData node_data(const ms_ctx* c) {
     NodeData nd{};
     nd.N = c->N;
     nd.mass = c->mass;
+    nd.ref = c->ref;
     nd.clamp = c->clamp;
     nd.csr_off = c->csr_off;
     nd.csr_ent = c->csr_ent;
@@ -545,6 +559,43 @@ int grid_for(const ms_ctx* c, int nchunks, int per_cu) {
     int g = c->num_cus * per_cu;
     g = ((g + 7) / 8) * 8;
     return std::max(1, std::min(g, nchunks));
+}
+
+// Nodes per tile (node group) under tuning n.  The balanced choice (-2): when
+// full tiles are fewer than the launch's workgroups G, some workgroups (and
+// CUs) would sit idle, so the tiles are cut to the smallest size (at least a
+// wave's worth of nodes, 64) whose count still fits one round of G.  With at
+// least G full tiles it keeps them: cutting them to fill whole rounds
+// (the 1.8M-triangle disc: 1774 tiles of 512 nodes on 512 workgroups, 3.46
+// rounds, against 2046 of 444 in 4) measured 6-10% slower, since a partial
+// last round runs faster and smaller tiles recompute more halo; filling one
+// round (the 290k-triangle disc: 284 tiles of 512 nodes, against 512 of 285)
+// measured 1.3x faster.  G is the persistent grid, or (one workgroup per
+// tile) the workgroups the kernel's registers let each CU hold.
+int resolve_tile_nodes(const ms_ctx* c, const ms_tuning& n) {
+    const int full = std::max(1, std::min(MAX_TILE_CHUNKS, n.tile_chunks)) * n.node_block;
+    if (n.tile_nodes > 0) return std::min(n.tile_nodes, full);
+    if (n.tile_nodes != -2 || n.fused != 1) return full;
+    auto count = [&](int L) {
+        long long k = 0;
+        for (const ShellDev& S : c->hs) k += (S.num_nodes + L - 1) / L;
+        return k;
+    };
+    const bool mixed = c->precision == MS_MIXED;
+    const long long G = n.node_grid_per_cu > 0
+                            ? (long long)grid_for(c, INT_MAX, n.node_grid_per_cu)
+                            : (long long)c->num_cus * std::max(1, (mixed ? 1024 : 512) / n.node_block);
+    if (count(full) >= G) return full;
+    int lo = std::min(full, 64), hi = full;  // count() does not increase with L; count(full) < G
+    const long long cap = G;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (count(mid) <= cap)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
 }
 Launch tri_launch(const ms_ctx* c) {
     return Launch{grid_for(c, c->n_tri_chunks, c->tune.tri_grid_per_cu), c->tune.tri_block, c->stream};
@@ -711,8 +762,8 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     bool any_lce = false, any_a0 = false, any_b0 = false;
     for (int s = 0; s < num_shells; ++s) {
         const ms_shell_desc& d = shells[s];
-        if (d.num_nodes <= 0 || d.num_tris <= 0 || !d.patch_ids || !d.G || !d.H || !d.ref_areas || !d.thickness ||
-            !d.shear_modulus || !d.abar_info || !d.node_masses)
+        if (d.num_nodes <= 0 || d.num_tris <= 0 || !d.patch_ids || !d.G || !d.H || !d.ref_pos || !d.ref_areas ||
+            !d.thickness || !d.shear_modulus || !d.abar_info || !d.node_masses)
             return fail(nullptr, MS_ERR_ARG, fmt("ms_create: shell %d has a missing array", s));
         const bool lce = d.params.dial_protocol == MS_DIAL_LCE;
         if (!lce && (d.abar_info_cols != 3 || !d.bbar_info || d.bbar_info_cols != 3))
@@ -745,6 +796,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     c->tune.graph_steps = 16;
     c->tune.fused = 0;
     c->tune.tile_chunks = 1;
+    c->tune.tile_nodes = -1;
     c->acols = acols;
     int dev = 0;
     hipGetDevice(&dev);
@@ -768,7 +820,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     std::vector<int> pid(6 * (size_t)T);
     std::vector<double> G(6 * (size_t)T), H(18 * (size_t)T), ai((size_t)acols * T, 0.0), bi(3 * (size_t)T, 0.0);
     std::vector<double> a0(any_a0 ? 3 * (size_t)T : 0), b0(any_b0 ? 3 * (size_t)T : 0);
-    std::vector<double> hh(T), mm(T), ar(T), mass(N);
+    std::vector<double> hh(T), mm(T), ar(T), mass(N), rxy(2 * (size_t)N);
     std::vector<unsigned char> cl(N, 0);
     c->hs.assign(num_shells, ShellDev{});
     int tb = 0, nb = 0;
@@ -833,6 +885,8 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
             ar[g] = d.ref_areas[t];
         }
         for (int n = 0; n < d.num_nodes; ++n) {
+            rxy[(size_t)nb + n] = d.ref_pos[n];
+            rxy[(size_t)N + nb + n] = d.ref_pos[(size_t)d.num_nodes + n];
             mass[(size_t)nb + n] = d.node_masses[n];
             cl[(size_t)nb + n] = d.constrained ? (d.constrained[n] ? 1 : 0) : 0;
         }
@@ -874,6 +928,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     UP(c->mu, mm);
     UP(c->area, ar);
     UP(c->mass, mass);
+    UP(c->ref, rxy);
     UP(c->clamp, cl);
     UP(c->csr_off, off);
     UP(c->csr_ent, ent);
@@ -911,7 +966,7 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
     }
     hipMemset(c->b.F, 0, 3 * sizeof(double) * (size_t)N);
     hipMemset(c->d_clock, 0, sizeof(ClockDev));
-    if (build_chunks(c, c->tune.tri_block, c->tune.node_block) != MS_OK) {
+    if (build_chunks(c, c->tune.tri_block, c->tune.node_block, resolve_tile_nodes(c, c->tune)) != MS_OK) {
         set_last_error(c->err);
         return bail(MS_ERR_HIP);
     }
@@ -1628,13 +1683,21 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
             return fail(c, MS_ERR_ARG, fmt("tile_chunks must be 1 to %d (0 keeps)", MAX_TILE_CHUNKS));
         n.tile_chunks = t->tile_chunks;
     }
-    const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block;
+    if (t->tile_nodes) {
+        if (t->tile_nodes < -2) return fail(c, MS_ERR_ARG, "tile_nodes must be positive, -1, -2 or 0 (keeps)");
+        n.tile_nodes = t->tile_nodes;
+    }
+    if (n.tile_nodes > n.tile_chunks * n.node_block)
+        return fail(c, MS_ERR_ARG,
+                    fmt("tile_nodes %d exceeds tile_chunks * node_block = %d", n.tile_nodes, n.tile_chunks * n.node_block));
+    const int L = resolve_tile_nodes(c, n);
+    const bool rechunk = n.tri_block != c->tune.tri_block || n.node_block != c->tune.node_block || L != c->tile_L;
     const bool retile = rechunk || n.tile_chunks != c->tune.tile_chunks;
     if (int r = pull_shells(c)) return r;
     destroy_graph(c);
     if (rechunk) {
         // on failure the context keeps its current tables, tiles and tuning
-        if (int r = build_chunks(c, n.tri_block, n.node_block)) return r;
+        if (int r = build_chunks(c, n.tri_block, n.node_block, L)) return r;
         if (int r = push_shells(c)) return r;
     }
     c->tune = n;
@@ -1653,6 +1716,7 @@ extern "C" int ms_set_tuning(ms_ctx* c, const ms_tuning* t) {
 extern "C" int ms_get_tuning(const ms_ctx* c, ms_tuning* t) {
     if (!c || !t) return MS_ERR_ARG;
     *t = c->tune;
+    t->tile_nodes = c->tile_L;  // the size in force
     return MS_OK;
 }
 

@@ -26,6 +26,7 @@ struct TriData {
     const double* mu;         // T    def_shear_moduli
     const double* area;       // T    ref_area
     const float* hmuA32;      // 3T   (h, mu, area) in FP32 for mixed mode
+    const double* ref;        // 2N   reference x, y of the nodes (SoA, stride N)
 };
 
 struct NodeData {
@@ -34,6 +35,7 @@ struct NodeData {
     const unsigned char* clamp;  // N (1 = constrained)
     const long long* csr_off; // N + 1   patch-incidence CSR offsets
     const int* csr_ent;       // entries t*6 + slot, ascending per node
+    const double* ref;        // 2N reference x, y (SoA, stride N)
 };
 
 struct Buffers {
@@ -184,11 +186,12 @@ struct TileData {
 void launch_tile_gather(hipStream_t s, const void* in, void* out, int elem_bytes, int ncomp, int T, int Tp, int slot,
                         const int* tris);
 // Dynamic LDS of a tile launch: positions of the tile's own and halo nodes
-// (24 B each), in mixed mode the own nodes' running sums (24 B each), one
-// pass of contributions (3 values per entry), and the tile's per-pass entry
-// offsets (16-bit, max_passes rows of poff_stride).
+// (24 B each; in FP64 also their reference x, y, 16 B), in mixed mode the own
+// nodes' running sums (24 B each), one pass of contributions (3 values per
+// entry), and the tile's per-pass entry offsets (16-bit, max_passes rows of
+// poff_stride).
 inline int tile_lds_bytes(const TileData& tl, bool mixed) {
-    return 24 * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8) +
+    return (mixed ? 24 : 40) * tl.max_nodes + (mixed ? 24 * tl.poff_stride : 0) + tl.max_entries * 3 * (mixed ? 4 : 8) +
            ((2 * tl.max_passes * tl.poff_stride + 7) & ~7);
 }
 // Raise the dynamic-LDS limit of the tile kernels where a tile needs > 64 KiB.

@@ -115,7 +115,8 @@ void tile_step_kernel(TileKArgs k) {
     const int B = blockDim.x;
     const int stride = k.tl.poff_stride;
     double* lpos = smem;
-    double* lsum = smem + 3 * (size_t)k.tl.max_nodes;
+    double* lref = smem + 3 * (size_t)k.tl.max_nodes;  // FP64: reference x, y
+    double* lsum = smem + (MIXED ? 3 : 5) * (size_t)k.tl.max_nodes;
     CT* lcon = reinterpret_cast<CT*>(MIXED ? lsum + 3 * (size_t)stride : lsum);
     unsigned short* lpo = reinterpret_cast<unsigned short*>(lcon + 3 * (size_t)k.tl.max_entries);
     const int nt = k.tl.active ? k.tl.nactive : k.tl.ntiles;
@@ -217,6 +218,10 @@ void tile_step_kernel(TileKArgs k) {
             lpos[3 * i] = ld(X, g);
             lpos[3 * i + 1] = ld(X + N, g);
             lpos[3 * i + 2] = ld(X + 2 * (size_t)N, g);
+            if (!MIXED) {
+                lref[2 * i] = ld(k.nd.ref, g);
+                lref[2 * i + 1] = ld(k.nd.ref + N, g);
+            }
         }
         {
             // by 32-bit words: the table's stride is even
@@ -250,7 +255,7 @@ void tile_step_kernel(TileKArgs k) {
                     unsigned lp[6], dw[3] = {ndw[0], ndw[1], ndw[2]};
                     unpack(nlw, lp);
                     double out[18];
-                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, out);
+                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, TileRef{lref, lp}, out);
                     store(dw, reinterpret_cast<const CT*>(out));
                     if (jn + B < j1) fetch(jn + B);
                 }

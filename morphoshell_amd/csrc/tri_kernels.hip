@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256) void tri_forces_kernel(TriKArgs k) {
             for (int i = 0; i < 9; ++i) dst[i] = make_float2(out[2 * i], out[2 * i + 1]);
         } else {
             double out[18];
-            tri_forces_fp64<LCE, UNI, 0>(k.td, tp, (unsigned)t, pos, out);
+            const GlobalRef ref{k.td.ref, (size_t)k.td.N, p};
+            tri_forces_fp64<LCE, UNI, 0>(k.td, tp, (unsigned)t, pos, ref, out);
             double2* dst = reinterpret_cast<double2*>(k.contrib + (size_t)t * 18);
 #pragma unroll
             for (int i = 0; i < 9; ++i) dst[i] = make_double2(out[2 * i], out[2 * i + 1]);

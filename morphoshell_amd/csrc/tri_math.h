@@ -3,7 +3,10 @@
 //
 // For triangle t: gather its six patch nodes, form R_x, R_y (G applied to
 // the three vertices) and R_xx, R_xy, R_yy (H applied to the six patch
-// nodes), the metric a, the normal n and the curvature b, the programmed
+// nodes), both as weights of the edge vectors to patch node 0 (the weights
+// of each derivative sum to zero; in FP64 G and the vertex rows of H come
+// from the reference positions, see RecF64), the metric
+// a, the normal n and the curvature b, the programmed
 // abar, bbar at the shell's dial factor, and the 18 force contributions
 // -A dW/dx of the six patch nodes.  The energy densities and gradient are
 // the reference's (calc_energy_densities.cpp:62-77,
@@ -245,14 +248,48 @@ struct TilePos {
         z = q[2];
     }
 };
+// Reference x, y of patch node s: GlobalRef from the 2N SoA stream,
+// TileRef from the tile's LDS copy (2 doubles per tile-local node).
+struct GlobalRef {
+    const double* R;
+    size_t N;
+    const unsigned* p;
+    __device__ __forceinline__ void operator()(int s, double& x, double& y) const {
+        x = ld(R, p[s]);
+        y = ld(R + N, p[s]);
+    }
+};
+struct TileRef {
+    const double* lds;
+    const unsigned* p;
+    __device__ __forceinline__ void operator()(int s, double& x, double& y) const {
+        const double* q = lds + 2 * p[s];
+        x = q[0];
+        y = q[1];
+    }
+};
 
-// The FP64 record of triangle t: patch coefficients, dial data, h, mu and
-// area.  Loaded whole before any use, so that a triangle's record costs one
-// memory round trip (uniform branches, e.g. dial from ansatz, would otherwise
-// split the loads into serial round trips), and so that a caller can have
-// the next triangle's record in flight while it evaluates this one.
+// The FP64 record of triangle t: the second-derivative weights of its three
+// non-vertex patch nodes, dial data, h, mu and area.  Loaded whole before any
+// use, so that a triangle's record costs one memory round trip (uniform
+// branches, e.g. dial from ansatz, would otherwise split the loads into
+// serial round trips), and so that a caller can have the next triangle's
+// record in flight while it evaluates this one.
+//
+// The other 15 of the 24 patch coefficients follow from the reference
+// positions, which the kernels stage per node.  G is rows 1, 2 of
+// inv([1, x_v - c_x, y_v - c_y]) (calc_deriv_mats.cpp:70-91): in edge-vector
+// form (weights of P_1 - P_0 and P_2 - P_0) it is the inverse of the 2x2
+// matrix E of reference edge vectors (xi_v, eta_v) = (X_v - X_0, Y_v - Y_0),
+// v = 1, 2.  H interpolates quadratics on the six patch nodes
+// (set_up_patch_fitting.cpp:206-232), so each of its columns annihilates 1,
+// X and Y: sum_n H[n] = 0 and sum_n H[n] (xi_n, eta_n) = 0.  Given the rows of
+// nodes 3..5 these fix the vertex rows: (H_1, H_2) = -E^-T (t1, t2) with
+// t = sum_{n>=3} H[n] (xi_n, eta_n), and H_0 = -(H_1 + H_2 + sum_{n>=3} H[n]).
+// The reconstruction agrees with the host's G and H to ~4e-16 relative.
 struct RecF64 {
-    double gc[6], hc[18], h, mu, A;
+    double hc[9];  // H[n][k] of patch nodes n = 3..5 at 3 (n - 3) + k
+    double h, mu, A;
     DialRec dr;
 };
 
@@ -260,9 +297,7 @@ template <bool LCE, bool UNI, int CS = 0>
 __device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams& sp, unsigned t, RecF64& r) {
     const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) r.gc[k] = ld_stream<CS != 0>(td.G + k * T, t);
-#pragma unroll
-    for (int k = 0; k < 18; ++k) r.hc[k] = ld_stream<CS != 0>(td.H + k * T, t);
+    for (int k = 0; k < 9; ++k) r.hc[k] = ld_stream<CS != 0>(td.H + (9 + k) * T, t);
     r.h = UNI ? sp.h : ld_stream<CS != 0>(td.h, t);
     r.mu = UNI ? sp.mu : ld_stream<CS != 0>(td.mu, t);
     r.A = ld_stream<CS != 0>(td.area, t);
@@ -270,43 +305,74 @@ __device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams
 }
 
 // FP64 force contributions (out[3*slot + c]) of the triangle whose record is
-// r; pos fetches its six patch nodes.
-template <bool LCE, class Pos>
+// r; pos fetches its six patch nodes, ref their reference x, y.
+template <bool LCE, class Pos, class Ref>
 __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriParams& sp, const Pos& pos,
-                                                    double out[18]) {
+                                                    const Ref& ref, double out[18]) {
 #pragma clang fp contract(on)
-    const double* gc = r.gc;
     const double* hc = r.hc;
     const double h = r.h, mu = r.mu, A = r.A;
     const double nu = sp.nu;
-    double Px[6], Py[6], Pz[6];
+    // reference edge vectors to patch node 0, and the first-derivative
+    // weights gc of P_1 - P_0, P_2 - P_0: (G[0][1], G[0][2], G[1][1], G[1][2])
+    double xi[6], eta[6];
+    {
+        double X0, Y0;
+        ref(0, X0, Y0);
 #pragma unroll
-    for (int s = 0; s < 6; ++s) pos(s, Px[s], Py[s], Pz[s]);
-    double Rx[3] = {0.0, 0.0, 0.0}, Ry[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        const double g0 = gc[v], g1 = gc[3 + v];
-        Rx[0] += g0 * Px[v];
-        Rx[1] += g0 * Py[v];
-        Rx[2] += g0 * Pz[v];
-        Ry[0] += g1 * Px[v];
-        Ry[1] += g1 * Py[v];
-        Ry[2] += g1 * Pz[v];
+        for (int s = 1; s < 6; ++s) {
+            double X, Y;
+            ref(s, X, Y);
+            xi[s] = X - X0;
+            eta[s] = Y - Y0;
+        }
     }
-    double R2[3][3];  // R2[k][c]: k = xx, xy, yy
+    const double idet = 1.0 / (xi[1] * eta[2] - xi[2] * eta[1]);
+    const double gc[4] = {eta[2] * idet, -eta[1] * idet, -xi[2] * idet, xi[1] * idet};
+    // the vertex rows of H from the moment identities
+    double hv[3][3];  // hv[v][k], v = 0, 1, 2
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
-        double sx = 0.0, sy = 0.0, sz = 0.0;
+        const double s0 = (hc[kk] + hc[3 + kk]) + hc[6 + kk];
+        const double t1 = hc[kk] * xi[3] + hc[3 + kk] * xi[4] + hc[6 + kk] * xi[5];
+        const double t2 = hc[kk] * eta[3] + hc[3 + kk] * eta[4] + hc[6 + kk] * eta[5];
+        hv[1][kk] = -(gc[0] * t1 + gc[2] * t2);
+        hv[2][kk] = -(gc[1] * t1 + gc[3] * t2);
+        hv[0][kk] = -((hv[1][kk] + hv[2][kk]) + s0);
+    }
+    double x0, y0, z0;
+    pos(0, x0, y0, z0);
+    // first and second derivatives from the edge vectors to patch node 0
+    double Rx[3], Ry[3], R2[3][3];  // R2[k][c]: k = xx, xy, yy
 #pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            const double hv = hc[s * 3 + kk];
-            sx += hv * Px[s];
-            sy += hv * Py[s];
-            sz += hv * Pz[s];
+    for (int s = 1; s < 6; ++s) {
+        double px, py, pz;
+        pos(s, px, py, pz);
+        const double e[3] = {px - x0, py - y0, pz - z0};
+        if (s == 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Rx[c] = gc[0] * e[c];
+                Ry[c] = gc[2] * e[c];
+            }
+        } else if (s == 2) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Rx[c] += gc[1] * e[c];
+                Ry[c] += gc[3] * e[c];
+            }
         }
-        R2[kk][0] = sx;
-        R2[kk][1] = sy;
-        R2[kk][2] = sz;
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const double w = s < 3 ? hv[s][kk] : hc[3 * (s - 3) + kk];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (s == 1)
+                    R2[kk][c] = w * e[c];
+                else
+                    R2[kk][c] += w * e[c];
+            }
+        }
     }
     const double a0 = Rx[0] * Rx[0] + Rx[1] * Rx[1] + Rx[2] * Rx[2];
     const double a1 = Rx[0] * Ry[0] + Rx[1] * Ry[1] + Rx[2] * Ry[2];
@@ -353,25 +419,29 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
         Agy[c] = A * (Wa[1] * Rx[c] + 2.0 * Wa[2] * Ry[c] + cy[c]);
         An[c] = A * n[c];
     }
+    // node 0's first-derivative weights: minus the sums of the others'
+    const double g0[2] = {-(gc[0] + gc[1]), -(gc[2] + gc[3])};
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const double ws = hc[s * 3] * Wb[0] + hc[s * 3 + 1] * Wb[1] + hc[s * 3 + 2] * Wb[2];
+        const double* hs = s < 3 ? hv[s] : hc + 3 * (s - 3);
+        const double ws = hs[0] * Wb[0] + hs[1] * Wb[1] + hs[2] * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double f = ws * An[c];
-            if (s < 3) f -= gc[s] * Agx[c] + gc[3 + s] * Agy[c];
+            if (s == 0) f -= g0[0] * Agx[c] + g0[1] * Agy[c];
+            if (s == 1 || s == 2) f -= gc[s - 1] * Agx[c] + gc[s + 1] * Agy[c];
             out[3 * s + c] = f;
         }
     }
 }
 
 // FP64 force contributions of triangle t: its record, then the mechanics.
-template <bool LCE, bool UNI, int CS, class Pos>
+template <bool LCE, bool UNI, int CS, class Pos, class Ref>
 __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
-                                                double out[18]) {
+                                                const Ref& ref, double out[18]) {
     RecF64 r;
     load_rec_fp64<LCE, UNI, CS>(td, sp, t, r);
-    tri_forces_fp64_rec<LCE>(r, sp, pos, out);
+    tri_forces_fp64_rec<LCE>(r, sp, pos, ref, out);
 }
 
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
@@ -487,14 +557,21 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
         Agy[c] = A * (Wa[1] * Rx[c] + 2.0f * Wa[2] * Ry[c] + cy[c]);
         An[c] = A * n[c];
     }
+    // node 0's weights: minus the sums of the others' (G and H annihilate
+    // constants), so the record's node-0 coefficients are never read
+    const float g0[2] = {-(g[0] + g[1]), -(g[2] + g[3])};
+    float h0[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) h0[kk] = -(((hc[0][kk] + hc[1][kk]) + (hc[2][kk] + hc[3][kk])) + hc[4][kk]);
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const float ws = ld(td.H32 + (s * 3) * T, t) * Wb[0] + ld(td.H32 + (s * 3 + 1) * T, t) * Wb[1] +
-                         ld(td.H32 + (s * 3 + 2) * T, t) * Wb[2];
+        const float* hs = s == 0 ? h0 : hc[s - 1];
+        const float ws = hs[0] * Wb[0] + hs[1] * Wb[1] + hs[2] * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             float f = ws * An[c];
-            if (s < 3) f -= ld(td.G32 + s * T, t) * Agx[c] + ld(td.G32 + (3 + s) * T, t) * Agy[c];
+            if (s == 0) f -= g0[0] * Agx[c] + g0[1] * Agy[c];
+            if (s == 1 || s == 2) f -= g[s - 1] * Agx[c] + g[s + 1] * Agy[c];
             out[3 * s + c] = f;
         }
     }

@@ -406,6 +406,7 @@ int run(int argc, char** argv) {
     d.patch_ids = r.pid.data();
     d.G = r.G.data();
     d.H = r.H.data();
+    d.ref_pos = r.mesh.pos.data();
     d.ref_areas = r.areas.data();
     d.thickness = r.mesh.thickness.data();
     d.shear_modulus = r.mesh.shear_modulus.data();
@@ -424,8 +425,11 @@ int run(int argc, char** argv) {
     // Launch shape by mesh size, from measured step times on discs of 18k to
     // 7.3M triangles: small meshes are launch-bound and fastest as a triangle
     // and a node launch of 64-wide blocks; from ~100k triangles the fused step
-    // on 256-node tiles; from ~1M triangles on 512-node tiles (enough tiles to
-    // fill the device).  Every shape gives the same bits.
+    // on tiles of up to 512 nodes (two 256-node chunks in FP64, one chunk in
+    // mixed mode) on a persistent grid of two workgroups per CU, cut smaller
+    // while they are fewer than the grid's workgroups (the 290k-triangle
+    // disc: 512 tiles of 285 nodes, 1.3x faster than 284 of 512).  Every
+    // shape gives the same bits.
     ms_tuning tune{};
     tune.tri_grid_per_cu = -1;
     tune.node_grid_per_cu = -1;
@@ -433,11 +437,10 @@ int run(int argc, char** argv) {
         tune.tri_block = tune.node_block = 64;
         tune.tri_grid_per_cu = tune.node_grid_per_cu = 8;
         tune.fused = 0;
-    } else if (T < 1000000) {
-        tune.node_block = 256;
-        tune.fused = 1;
     } else {
-        tune.node_block = 512;
+        tune.tile_nodes = -2;
+        tune.node_block = precision == MS_MIXED ? 512 : 256;
+        tune.tile_chunks = precision == MS_MIXED ? 1 : 2;
         tune.node_grid_per_cu = 2;
         tune.fused = 1;
     }

@@ -90,7 +90,7 @@ class NNGraphDesc(C.Structure):
 class ShellDesc(C.Structure):
     _fields_ = [
         ("num_nodes", C.c_int32), ("num_tris", C.c_int32), ("patch_ids", _ip), ("G", _dp), ("H", _dp),
-        ("ref_areas", _dp), ("thickness", _dp), ("shear_modulus", _dp), ("abar_info", _dp),
+        ("ref_pos", _dp), ("ref_areas", _dp), ("thickness", _dp), ("shear_modulus", _dp), ("abar_info", _dp),
         ("abar_info_cols", C.c_int32), ("bbar_info_cols", C.c_int32), ("bbar_info", _dp), ("a0", _dp), ("b0", _dp),
         ("node_masses", _dp), ("constrained", _u8p), ("params", ShellParams),
     ]
@@ -112,7 +112,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("tri_block", C.c_int32), ("node_block", C.c_int32), ("tri_grid_per_cu", C.c_int32),
         ("node_grid_per_cu", C.c_int32), ("graph_steps", C.c_int32), ("fused", C.c_int32),
-        ("tile_chunks", C.c_int32), ("reserved", C.c_int32),
+        ("tile_chunks", C.c_int32), ("tile_nodes", C.c_int32),
     ]
 
 

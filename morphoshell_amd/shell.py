@@ -308,6 +308,8 @@ class Context:
             d.patch_ids = nat.iptr(keep["pid"])
             d.G = nat.dptr(keep["G"])
             d.H = nat.dptr(keep["H"])
+            keep["rp"] = np.ascontiguousarray(sh.ref_pos, dtype=np.float64)
+            d.ref_pos = nat.dptr(keep["rp"])
             d.ref_areas = nat.dptr(sh.areas)
             d.thickness = nat.dptr(sh.thickness)
             d.shear_modulus = nat.dptr(sh.shear_modulus)
@@ -546,12 +548,13 @@ class Context:
         return int(self.L.ms_steps_taken(self.h))
 
     def set_tuning(self, tri_block=0, node_block=0, tri_grid_per_cu=-1, node_grid_per_cu=-1, graph_steps=0,
-                   fused=-1, tile_chunks=0):
+                   fused=-1, tile_chunks=0, tile_nodes=0):
         t = nat.Tuning()
         t.tri_block, t.node_block = tri_block, node_block
         t.tri_grid_per_cu, t.node_grid_per_cu, t.graph_steps = tri_grid_per_cu, node_grid_per_cu, graph_steps
         t.fused = fused
         t.tile_chunks = tile_chunks
+        t.tile_nodes = tile_nodes
         self._ck(self.L.ms_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self) -> dict:
@@ -559,7 +562,7 @@ class Context:
         self._ck(self.L.ms_get_tuning(self.h, C.byref(t)))
         return {"tri_block": t.tri_block, "node_block": t.node_block, "tri_grid_per_cu": t.tri_grid_per_cu,
                 "node_grid_per_cu": t.node_grid_per_cu, "graph_steps": t.graph_steps, "fused": t.fused,
-                "tile_chunks": t.tile_chunks}
+                "tile_chunks": t.tile_chunks, "tile_nodes": t.tile_nodes}
 
     def enable_timing(self, on: bool = True) -> None:
         self._ck(self.L.ms_enable_timing(self.h, 1 if on else 0))

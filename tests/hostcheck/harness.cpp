@@ -49,6 +49,7 @@ struct Shell {
         d.patch_ids = pid.data();
         d.G = G.data();
         d.H = H.data();
+        d.ref_pos = pos.data();
         d.ref_areas = area.data();
         d.thickness = h.data();
         d.shear_modulus = mu.data();
@@ -136,8 +137,9 @@ Shell make_shell(int n, unsigned seed, bool lce = false) {
     return s;
 }
 
-ms_tuning tuning(int tb, int nb, int tg, int ng, int gs, int fused, int chunks) {
+ms_tuning tuning(int tb, int nb, int tg, int ng, int gs, int fused, int chunks, int tile_nodes = 0) {
     ms_tuning t{};
+    t.tile_nodes = tile_nodes;
     t.tri_block = tb;
     t.node_block = nb;
     t.tri_grid_per_cu = tg;
@@ -199,11 +201,14 @@ void scenario_small_shapes() {
             OK(ms_step(c, 5, 0, 0.0, 0.0, 0, rep.data()));
             OK(ms_enable_timing(c, 0));
         }
-        for (int nb : {128, 256, 512}) {
-            ms_tuning t = tuning(256, nb, 0, 2, 16, 1, nb == 512 ? 1 : 2);
-            OK(ms_set_tuning(c, &t));
-            OK(ms_step(c, 40, 0, 0.0, 0.0, 0, nullptr));
-        }
+        for (int nb : {128, 256, 512})
+            for (int tn : {-1, -2, 100, -1}) {
+                // full tiles, balanced over the grid, 100-node groups of
+                // partial chunks, and back
+                ms_tuning t = tuning(256, nb, 0, 2, 16, 1, nb == 512 ? 1 : 2, tn);
+                OK(ms_set_tuning(c, &t));
+                OK(ms_step(c, 40, 0, 0.0, 0.0, 0, nullptr));
+            }
         OK(ms_set_precision(c, prec == MS_FP64 ? MS_MIXED : MS_FP64));
         OK(ms_step(c, 20, 0, 0.0, 0.0, 0, nullptr));
         exercise_ops(c, sh);
@@ -225,7 +230,7 @@ void scenario_batch_halts() {
                 ms_ctx* c = nullptr;
                 OK(ms_create(d.data(), (int)d.size(), MS_MIXED, &c));
                 set_states(c, sh);
-                ms_tuning t = tuning(64, 128, grid, grid, 8, fused, chunks);
+                ms_tuning t = tuning(64, 128, grid, grid, 8, fused, chunks, chunks == 2 ? -2 : -1);
                 OK(ms_set_tuning(c, &t));
                 mock_clear_halts();
                 mock_schedule_halt(1, 30);

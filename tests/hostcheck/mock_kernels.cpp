@@ -93,6 +93,7 @@ void check_tri_streams(const ms::TriData& td, bool mixed, bool lce, bool uni, co
 void check_node_streams(const ms::NodeData& nd, const ms::Buffers& b, const char* what) {
     const long long N = nd.N;
     dev(nd.mass, N, WHAT(what, nd.mass));
+    dev(nd.ref, 2LL * N, WHAT(what, nd.ref));
     dev(nd.clamp, N, WHAT(what, nd.clamp));
     dev(nd.csr_off, N + 1, WHAT(what, nd.csr_off));
     dev(nd.csr_ent, nd.csr_off[N], WHAT(what, nd.csr_ent));
@@ -141,6 +142,7 @@ void launch_tri_forces(const Launch& L, const TriData& td, const Buffers& b, con
         check_chunks(L, chunks, nchunks, shells, 0, true, w);
         check_tri_streams(td, mixed, lce, uni_data, w);
         for (int i = 0; i < 2; ++i) dev(b.x[i], 3LL * td.N, WHAT(w, b.x[i]));
+        if (!mixed) dev(td.ref, 2LL * td.N, WHAT(w, td.ref));
         if (mixed)
             dev(b.contrib32, 18LL * td.T, WHAT(w, b.contrib32));
         else
@@ -435,6 +437,8 @@ void launch_tile_step(const Launch& L, const NodeData& nd, const Buffers& b, con
                 }
             }
             for (int r = 0; r < MAX_TILE_CHUNKS && r * L.block < nn; ++r) {
+                if (ch.w + r < S.node_chunk_begin || ch.w + r >= S.node_chunk_begin + S.node_chunk_count)
+                    bad(w, "tile chunk outside its shell's node chunks", ci, ch.w + r);
                 dev(b.part + 2LL * (ch.w + r), 2, WHAT(w, b.part + 2LL * (ch.w + r)));
                 dev(b.part_i + 2LL * (ch.w + r), 2, WHAT(w, b.part_i + 2LL * (ch.w + r)));
             }

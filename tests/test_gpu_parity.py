@@ -815,7 +815,8 @@ def test_refused_tile_shape_leaves_a_working_two_launch_context():
     sh = _relabelled_shell()
     x, v = deformed_state(sh, seed=41)
     with _ctx(sh) as a, _ctx(sh) as b:
-        a.set_tuning(node_block=256, fused=1)
+        a.set_tuning(node_block=128, fused=1)
+        assert a.get_tuning()["fused"] == 1
         with pytest.raises(NativeError, match="LDS"):
             a.set_tuning(node_block=512, fused=1, tile_chunks=2)
         t = a.get_tuning()
@@ -846,6 +847,36 @@ def test_two_chunk_fp64_tiles_take_short_passes_and_match_two_launch(sh, node_bl
         assert np.array_equal(a.get_state(0)[1], b.get_state(0)[1])
         assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
         assert ra[0]["max_speed_ratio"] == rb[0]["max_speed_ratio"]
+
+
+@pytest.mark.parametrize("tile_nodes", [-2, 333, 100])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_tiles_of_any_size_step_like_two_launch(precision, tile_nodes):
+    """Tiles smaller than their chunks: the size balanced over a persistent
+    grid (here the 64-node floor: two small shells give fewer full tiles than
+    the grid has workgroups) and explicit sizes that cut a partial second
+    chunk (333 = 256 + 77) or one partial chunk (100).  Node groups restart
+    at each shell.  The fused step stays bit-identical to the two-launch
+    form, the equilibrium check included, and get_tuning reports the size in
+    force."""
+    shells = [_shell(0.02, seed=5), _shell(0.03, seed=6, kappa=-0.4)]
+    states = [deformed_state(s, seed=60 + i) for i, s in enumerate(shells)]
+    with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        a.set_tuning(node_block=256, tile_chunks=2, fused=1, node_grid_per_cu=2, tile_nodes=tile_nodes)
+        t = a.get_tuning()
+        assert t["fused"] == 1
+        assert t["tile_nodes"] == (64 if tile_nodes == -2 else tile_nodes)
+        b.set_tuning(node_block=256, fused=0)
+        for c in (a, b):
+            for i, st in enumerate(states):
+                c.set_state(i, *st)
+        ra = a.step(24, equil_every=8)
+        rb = b.step(24, equil_every=8)
+        for i in range(len(shells)):
+            assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0])
+            assert np.array_equal(a.get_state(i)[1], b.get_state(i)[1])
+            assert ra[i]["max_force_ratio"] == rb[i]["max_force_ratio"]
+            assert ra[i]["max_speed_ratio"] == rb[i]["max_speed_ratio"]
 
 
 def test_wrapper_checks_array_sizes_and_shell_indices(sh):
