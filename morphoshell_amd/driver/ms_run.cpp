@@ -422,21 +422,31 @@ int run(int argc, char** argv) {
         r.log << "Error: " << ms_last_error() << std::endl;
         return -1;
     }
-    // Launch shape by mesh size, from measured step times on discs of 18k to
-    // 7.3M triangles: small meshes are launch-bound and fastest as a triangle
-    // and a node launch of 64-wide blocks; from ~100k triangles the fused step
-    // on tiles of up to 512 nodes (two 256-node chunks in FP64, one chunk in
-    // mixed mode) on a persistent grid of two workgroups per CU, cut smaller
-    // while they are fewer than the grid's workgroups (the 290k-triangle
-    // disc: 512 tiles of 285 nodes, 1.3x faster than 284 of 512).  Every
-    // shape gives the same bits.
+    // Launch shape by mesh size, from measured step times on discs of 4.5k to
+    // 7.3M triangles.  Every shape is the fused step and gives the same bits.
+    // Small meshes are launch-bound: many small tiles, one workgroup each, keep
+    // most CUs busy (4.5k triangles: 16-node tiles in 128-thread workgroups,
+    // 11.1 / 9.8 us per step in FP64 / mixed; 18k: 48-node tiles in 256-thread
+    // workgroups, 9.6 / 8.8 us; 72k: 48-node tiles, 13.2 / 10.7 us; against
+    // 12.4 / 11.9, 12.0 / 11.3 and 17.2 / 13.8 us for a triangle and a node
+    // launch of 64-wide blocks).  From ~100k triangles: tiles of up to 512
+    // nodes (two 256-node chunks in FP64, one chunk in mixed mode) on a
+    // persistent grid of two workgroups per CU, cut smaller while they are
+    // fewer than the grid's workgroups (the 290k-triangle disc: 512 tiles of
+    // 285 nodes, 1.35x faster than 284 of 512).
     ms_tuning tune{};
     tune.tri_grid_per_cu = -1;
     tune.node_grid_per_cu = -1;
-    if (T < 100000) {
-        tune.tri_block = tune.node_block = 64;
-        tune.tri_grid_per_cu = tune.node_grid_per_cu = 8;
-        tune.fused = 0;
+    tune.fused = 1;
+    tune.tile_chunks = 1;
+    if (T < 10000) {
+        tune.node_block = 128;
+        tune.tile_nodes = 16;
+        tune.node_grid_per_cu = 0;
+    } else if (T < 100000) {
+        tune.node_block = 256;
+        tune.tile_nodes = 48;
+        tune.node_grid_per_cu = 0;
     } else {
         tune.tile_nodes = -2;
         tune.node_block = precision == MS_MIXED ? 512 : 256;
