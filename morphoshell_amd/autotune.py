@@ -29,7 +29,8 @@ DEFAULT_SPACE = {
     "graph_steps": (16,),
     "fused_node_block": (128, 256, 512),  # block sizes of the fused step
     "fused_tile_chunks": (1, 2),      # node chunks per tile of the fused step
-    "fused_tile_nodes": (-1, -2),     # full tiles, or cut to fill the grid (same when they already do)
+    # balanced tiles (cut to fill the grid, and re-cut as shells halt), or full ones
+    "fused_tile_nodes": (-2, -1),
 }
 
 
@@ -130,7 +131,11 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
         return dict(best, cached=True)
     sp = space or DEFAULT_SPACE
     results = []
+    balanced_full = set()  # shapes whose balanced tiles are full ones: -1 is the same launch
     for cand in _candidates(sp):
+        key = tuple(sorted((k, v) for k, v in cand.items() if k != "tile_nodes"))
+        if cand.get("fused") and cand.get("tile_nodes") == -1 and key in balanced_full:
+            continue
         try:
             _apply(ctx, cand)
         except NativeError as e:  # a shape the library refuses for this context (e.g. tile LDS)
@@ -138,7 +143,7 @@ def tune(ctx: Context, steps: int = 40, space: dict | None = None, use_cache: bo
                 print(f"[autotune] {sig} {cand}: refused ({e})", file=sys.stderr)
             continue
         if cand.get("tile_nodes") == -2 and ctx.get_tuning()["tile_nodes"] == cand["tile_chunks"] * cand["node_block"]:
-            continue  # the tiles already fill the grid: the same shape as full tiles
+            balanced_full.add(key)
         t = time_config(ctx, steps)
         results.append((t, cand))
         if verbose:

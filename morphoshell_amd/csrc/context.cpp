@@ -572,13 +572,14 @@ int grid_for(const ms_ctx* c, int nchunks, int per_cu) {
 // round (the 290k-triangle disc: 284 tiles of 512 nodes, against 512 of 285)
 // measured 1.3x faster.  G is the persistent grid, or (one workgroup per
 // tile) the workgroups the kernel's registers let each CU hold.
-int resolve_tile_nodes(const ms_ctx* c, const ms_tuning& n) {
+int resolve_tile_nodes(const ms_ctx* c, const ms_tuning& n, const std::vector<char>* live = nullptr) {
     const int full = std::max(1, std::min(MAX_TILE_CHUNKS, n.tile_chunks)) * n.node_block;
     if (n.tile_nodes > 0) return std::min(n.tile_nodes, full);
     if (n.tile_nodes != -2 || n.fused != 1) return full;
     auto count = [&](int L) {
         long long k = 0;
-        for (const ShellDev& S : c->hs) k += (S.num_nodes + L - 1) / L;
+        for (int s = 0; s < c->num_shells; ++s)
+            if (!live || (*live)[s]) k += (c->hs[s].num_nodes + L - 1) / L;
         return k;
     };
     const bool mixed = c->precision == MS_MIXED;
@@ -645,6 +646,23 @@ int refresh_active(ms_ctx* c) {
     }
     if (live == c->act_live) return MS_OK;
     destroy_graph(c);
+    const bool any = std::find(live.begin(), live.end(), 1) != live.end();
+    if (c->tune.fused == 1 && c->tune.tile_nodes == -2 && any) {
+        // Balanced tiles follow the running shells: once they need tiles of
+        // at most 0.7x the current size (or, all running again, larger ones)
+        // every shell is re-cut.  The tables change which workgroups run, not
+        // the results.  Should the new tiles fail to build, the two-launch
+        // form (chunks only) carries on with the same results.
+        const int L = resolve_tile_nodes(c, c->tune, &live);
+        if (10 * (long long)L < 7LL * c->tile_L || (all && L > c->tile_L)) {
+            if (int r = build_chunks(c, c->tune.tri_block, c->tune.node_block, L)) return r;
+            if (int r = push_shells(c)) return r;
+            if (build_tiles(c) != MS_OK) {
+                c->tune.fused = 0;
+                c->err.clear();
+            }
+        }
+    }
     // the full tables until the new ones are in place (an upload error leaves
     // them in use and the running set unrecorded, so the next call retries)
     c->act_live.clear();

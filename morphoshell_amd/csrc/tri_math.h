@@ -444,13 +444,39 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
     tri_forces_fp64_rec<LCE>(r, sp, pos, ref, out);
 }
 
+// The mixed-mode record of triangle t: 19 of the 24 FP32 patch coefficients
+// (node 0's weights are minus the sums of the others'), h, mu, area and the
+// dial data.  Loaded whole before any use, as the FP64 record.
+struct RecMixed {
+    float g[4];       // G[0][1], G[0][2], G[1][1], G[1][2]
+    float hc[5][3];   // H[s][k] of patch nodes s = 1..5
+    float h, mu, A;
+    DialRec dr;
+};
+
+template <bool LCE, bool UNI, int CS = 0>
+__device__ __forceinline__ void load_rec_mixed(const TriData& td, const TriParams& sp, unsigned t, RecMixed& r) {
+    const size_t T = CS ? (size_t)CS : (size_t)td.T;
+    r.g[0] = ld(td.G32 + 1 * T, t);
+    r.g[1] = ld(td.G32 + 2 * T, t);
+    r.g[2] = ld(td.G32 + 4 * T, t);
+    r.g[3] = ld(td.G32 + 5 * T, t);
+#pragma unroll
+    for (int s = 1; s < 6; ++s)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) r.hc[s - 1][kk] = ld(td.H32 + (s * 3 + kk) * T, t);
+    r.h = UNI ? (float)sp.h : ld(td.hmuA32, t);
+    r.mu = UNI ? (float)sp.mu : ld(td.hmuA32 + T, t);
+    r.A = ld(td.hmuA32 + 2 * T, t);
+    load_dial<LCE, UNI, CS>(td, sp, t, r.dr);
+}
+
 // Mixed mode: edge vectors to vertex 0 and the strains a - abar, b - bbar in
 // double, the patch fit, stresses and contributions in float.
-template <bool LCE, bool UNI, int CS, class Pos>
-__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
-                                                 float out[18]) {
+template <bool LCE, class Pos>
+__device__ __forceinline__ void tri_forces_mixed_rec(const RecMixed& r, const TriParams& sp, const Pos& pos,
+                                                     float out[18]) {
 #pragma clang fp contract(on)
-    const size_t T = CS ? (size_t)CS : (size_t)td.T;
     // Edge vectors, the surface derivatives and the strains a - abar, b - bbar
     // are formed in FP64 (the FP32 patch-fit coefficients are exact in
     // double).  Rounding the tangent vectors to FP32 before forming the
@@ -458,20 +484,10 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
     // step, uncorrelated between steps); under the weak damping that noise
     // holds the residual force far above its static error.  The stresses
     // and the force assembly below run in FP32.
-    float g[4], hc[5][3];
-    g[0] = ld(td.G32 + 1 * T, t);
-    g[1] = ld(td.G32 + 2 * T, t);
-    g[2] = ld(td.G32 + 4 * T, t);
-    g[3] = ld(td.G32 + 5 * T, t);
-#pragma unroll
-    for (int s = 1; s < 6; ++s)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) hc[s - 1][kk] = ld(td.H32 + (s * 3 + kk) * T, t);
-    const float h = UNI ? (float)sp.h : ld(td.hmuA32, t);
-    const float mu = UNI ? (float)sp.mu : ld(td.hmuA32 + T, t);
-    const float A = ld(td.hmuA32 + 2 * T, t);
-    DialRec dr;
-    load_dial<LCE, UNI, CS>(td, sp, t, dr);
+    const float* g = r.g;
+    const float(*hc)[3] = r.hc;
+    const float h = r.h, mu = r.mu, A = r.A;
+    const DialRec& dr = r.dr;
     double x0, y0, z0;
     pos(0, x0, y0, z0);
     // one edge vector at a time into the first and second derivatives
@@ -575,6 +591,15 @@ __device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriPar
             out[3 * s + c] = f;
         }
     }
+}
+
+// Mixed-mode force contributions of triangle t: its record, then the mechanics.
+template <bool LCE, bool UNI, int CS, class Pos>
+__device__ __forceinline__ void tri_forces_mixed(const TriData& td, const TriParams& sp, unsigned t, const Pos& pos,
+                                                 float out[18]) {
+    RecMixed r;
+    load_rec_mixed<LCE, UNI, CS>(td, sp, t, r);
+    tri_forces_mixed_rec<LCE>(r, sp, pos, out);
 }
 
 }  // namespace ms

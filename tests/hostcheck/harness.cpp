@@ -250,6 +250,43 @@ void scenario_batch_halts() {
     std::printf("batch halts ok\n");
 }
 
+// Balanced tiles (tile_nodes -2) on a device of 1 CU: the running shells'
+// tiles are re-cut smaller as shells halt and full again when all run.
+void scenario_balanced_retile() {
+    setenv("MOCK_NUM_CUS", "1", 1);
+    std::vector<Shell> sh{make_shell(30, 11), make_shell(24, 12), make_shell(12, 13), make_shell(26, 14)};
+    std::vector<ms_shell_desc> d;
+    for (auto& s : sh) d.push_back(s.desc());
+    for (int prec : {MS_FP64, MS_MIXED}) {
+        ms_ctx* c = nullptr;
+        OK(ms_create(d.data(), (int)d.size(), prec, &c));
+        set_states(c, sh);
+        ms_tuning t = tuning(64, 128, 0, 0, 8, 1, 2, -2);
+        OK(ms_set_tuning(c, &t));
+        ms_tuning got{};
+        OK(ms_get_tuning(c, &got));
+        REQUIRE(got.tile_nodes == 256, "balanced tiles on 1 CU: %d nodes, expected full 256", got.tile_nodes);
+        mock_clear_halts();
+        mock_schedule_halt(0, 20);
+        mock_schedule_halt(1, 20);
+        mock_schedule_halt(3, 20);
+        std::vector<ms_step_report> rep(sh.size());
+        for (int k = 0; k < 6; ++k) OK(ms_step(c, 9, 0, 0.0, 0.0, 0, rep.data()));
+        OK(ms_step(c, 9, 0, 0.0, 0.0, 0, rep.data()));  // the halts are seen: re-cut
+        OK(ms_get_tuning(c, &got));
+        REQUIRE(got.tile_nodes < 256 && got.fused == 1, "running shell's tiles not re-cut: %d nodes", got.tile_nodes);
+        mock_clear_halts();
+        for (int s = 0; s < (int)sh.size(); ++s) OK(ms_resume_shell(c, s));
+        OK(ms_step(c, 9, 0, 0.0, 0.0, 0, rep.data()));
+        OK(ms_get_tuning(c, &got));
+        REQUIRE(got.tile_nodes == 256, "tiles not full again with every shell running: %d", got.tile_nodes);
+        exercise_ops(c, sh);
+        ms_destroy(c);
+    }
+    unsetenv("MOCK_NUM_CUS");
+    std::printf("balanced retile ok\n");
+}
+
 // Dial-from-ansatz geometry set while the fused step's graph is live, and the
 // device-run relaxation protocol.
 void scenario_zero_dial_and_relax() {
@@ -351,6 +388,7 @@ void scenario_alloc_failures() {
 int main() {
     scenario_small_shapes();
     scenario_batch_halts();
+    scenario_balanced_retile();
     scenario_zero_dial_and_relax();
     scenario_alloc_failures();
     scenario_noalloc();

@@ -903,17 +903,20 @@ def test_wrapper_checks_array_sizes_and_shell_indices(sh):
         assert np.array_equal(ctx.get_state(0)[0], x)
 
 
+@pytest.mark.parametrize("tile_nodes", [-1, -2])
 @pytest.mark.parametrize("tile_chunks", [1, 2])
 @pytest.mark.parametrize("grid_per_cu", [0, 2, 8])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_halted_shells_fused_active_tiles(precision, grid_per_cu, tile_chunks):
+def test_halted_shells_fused_active_tiles(precision, grid_per_cu, tile_chunks, tile_nodes):
     """With the fused form, after some shells halt the running shells' tiles
     run from the active-tile list, with a grid of one workgroup per tile
     (grid_per_cu 0) or a persistent grid of 2 or 8 workgroups per CU that
     strides over the list when it holds more tiles than the grid has
-    workgroups, and with tiles of one or two chunks.  The running
-    shells' states equal, bit for bit, those of a context where nothing halts
-    and that runs one workgroup per tile of one chunk."""
+    workgroups, and with tiles of one or two chunks.  With balanced tiles
+    (tile_nodes -2) the tiles are re-cut as shells halt (two-chunk tiles on
+    the grid of 8 per CU: 109 nodes while all 24 shells run, 64 once 12
+    do).  The running shells' states equal, bit for bit, those of a context
+    where nothing halts and that runs one workgroup per tile of one chunk."""
     # 24 shells of 9.2k nodes: 144 tiles each at 64-node tiles of one chunk
     # (72 at two chunks), so 3456 / 1728 tiles, 2880 / 1440 running after the
     # first halts and 1728 / 864 after the second; a grid of 2 per CU (512
@@ -922,8 +925,10 @@ def test_halted_shells_fused_active_tiles(precision, grid_per_cu, tile_chunks):
     shells = [_shell(0.02, seed=i, h=0.02, kappa=0.3 - 0.02 * i) for i in range(24)]
     states = [deformed_state(s, seed=300 + i) for i, s in enumerate(shells)]
     with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
-        a.set_tuning(node_block=64, tri_block=64, fused=1, node_grid_per_cu=grid_per_cu, tile_chunks=tile_chunks)
+        a.set_tuning(node_block=64, tri_block=64, fused=1, node_grid_per_cu=grid_per_cu, tile_chunks=tile_chunks,
+                     tile_nodes=tile_nodes)
         b.set_tuning(node_block=64, tri_block=64, fused=1, node_grid_per_cu=0, tile_chunks=1)
+        L0 = a.get_tuning()["tile_nodes"]
         assert a.get_tuning()["node_grid_per_cu"] == grid_per_cu
         assert a.get_tuning()["tile_chunks"] == tile_chunks
         for c in (a, b):
@@ -943,6 +948,9 @@ def test_halted_shells_fused_active_tiles(precision, grid_per_cu, tile_chunks):
             for i in running:
                 assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
                 assert np.array_equal(a.get_state(i)[1], b.get_state(i)[1]), i
+        if tile_nodes == -2 and tile_chunks == 2 and grid_per_cu == 8:
+            assert (L0, a.get_tuning()["tile_nodes"]) == (109, 64)
+            assert a.get_tuning()["fused"] == 1
 
 
 def test_cauchy_stress_matches_oracle(sh):
