@@ -211,6 +211,28 @@ void tile_step_kernel(TileKArgs k) {
             load_rec_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, nrec);
         };
         if (!MIXED && jn < j1) fetch(jn);
+        // FP64: the node state of the thread's nodes (velocity, mass, clamp
+        // flag) is loaded here, in flight through the tile, so that the
+        // update at its end waits on no memory (FP64 step 3-5% faster on
+        // discs of 18k to 1.8M triangles; the mixed kernel spills at its
+        // register limit and loads them at the update)
+        double pv[MAX_TILE_CHUNKS][3], pm[MAX_TILE_CHUNKS];
+        unsigned char pc[MAX_TILE_CHUNKS];
+        if constexpr (!MIXED) {
+            const double* V = k.b.v[par];
+#pragma unroll
+            for (int r = 0; r < MAX_TILE_CHUNKS; ++r) {
+                const int i = (int)threadIdx.x + r * B;
+                if (i < nn) {
+                    const int n = ch.x + i;
+                    pv[r][0] = __builtin_nontemporal_load(V + n);
+                    pv[r][1] = __builtin_nontemporal_load(V + N + n);
+                    pv[r][2] = __builtin_nontemporal_load(V + 2 * N + n);
+                    pm[r] = __builtin_nontemporal_load(k.nd.mass + n);
+                    pc[r] = k.nd.clamp[n];
+                }
+            }
+        }
         // 1. positions of the tile's nodes and its halo nodes, and the tile's
         // per-pass entry offsets
         for (int i = threadIdx.x; i < nn + (h1 - h0); i += B) {
@@ -315,15 +337,27 @@ void tile_step_kernel(TileKArgs k) {
                 // the node state is read and written once per step: non-temporal
                 // loads and stores leave the caches to the halo positions
                 // (measured: mixed step 3.4% faster, FP64 unchanged)
-                double vx = __builtin_nontemporal_load(V + n), vy = __builtin_nontemporal_load(V + N + n),
-                       vz = __builtin_nontemporal_load(V + 2 * N + n);
-                const double m = __builtin_nontemporal_load(k.nd.mass + n);
+                double vx, vy, vz, m;
+                unsigned char clamped;
+                if constexpr (MIXED) {
+                    vx = __builtin_nontemporal_load(V + n);
+                    vy = __builtin_nontemporal_load(V + N + n);
+                    vz = __builtin_nontemporal_load(V + 2 * N + n);
+                    m = __builtin_nontemporal_load(k.nd.mass + n);
+                    clamped = k.nd.clamp[n];
+                } else {
+                    vx = pv[r][0];
+                    vy = pv[r][1];
+                    vz = pv[r][2];
+                    m = pm[r];
+                    clamped = pc[r];
+                }
                 if (!sh.gd) {
                     gx = damped(gx, sh.eta, m, vx, sh.damp_den);
                     gy = damped(gy, sh.eta, m, vy, sh.damp_den);
                     gz = damped(gz, sh.eta, m, vz, sh.damp_den);
                 }
-                if (k.nd.clamp[n]) gx = gy = gz = 0.0;
+                if (clamped) gx = gy = gz = 0.0;
                 if (do_check) {
                     fnorm = norm3(gx, gy, gz);
                     vnorm = norm3(vx, vy, vz);
