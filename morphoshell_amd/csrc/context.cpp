@@ -75,6 +75,7 @@ struct ms_ctx {
     // fused tile step (built on demand for the current node chunks)
     TileData tile{};                 // device metadata + tile-ordered records
     int tile_T = 0;                  // total tile entries (triangles counted per tile)
+    long long tile_gen = 0;          // record generations built so far
     bool tiles_ready = false;
     std::vector<void*> tile_meta;    // metadata allocations
     std::vector<void*> tile_allocs;  // tile-ordered record copies
@@ -266,6 +267,7 @@ int build_tile_records(ms_ctx* c) {
     c->dev_bytes += c->tile_rec_bytes;
     c->tile.rec64 = (const double*)r64;
     c->tile.rec32 = (const float*)r32;
+    c->tile.gen = ++c->tile_gen;  // LCE metric caches of older records are void
     auto gather = [&](const void* in, void* out, int elem_bytes, int ncomp, int slot) {
         if (in) launch_tile_gather(c->stream, in, out, elem_bytes, ncomp, T, Tp, slot, c->tile.tris);
     };
@@ -517,6 +519,8 @@ int pull_shells(ms_ctx* c) {
     return MS_OK;
 }
 int push_shells(ms_ctx* c) {
+    // a record the host writes may change what the LCE metric cache holds
+    for (ShellDev& S : c->hs) S.lce_gen = -1;
     HIPC(c, hipStreamSynchronize(c->stream));
     HIPC(c, hipMemcpy(c->d_shells, c->hs.data(), c->hs.size() * sizeof(ShellDev), hipMemcpyHostToDevice));
     return MS_OK;
@@ -711,7 +715,7 @@ void step_once(ms_ctx* c, bool check, int halt_on_eq, double tol_f, double tol_v
     const bool mixed = c->precision == MS_MIXED;
     TriData td = tri_data(c);
     NodeData nd = node_data(c);
-    StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v};
+    StepFlags fl{check ? 1 : 0, halt_on_eq, tol_f, tol_v, c->tune.fused == 1 ? c->tile.gen : -1LL};
     if (timed) hipEventRecord(c->ev[0], c->stream);
     if (c->tune.fused == 1) {
         const TileData tld = tile_data(c);

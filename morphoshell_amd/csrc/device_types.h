@@ -76,6 +76,12 @@ struct ShellDev {
     int uni_b;             // bbar_info uniform: b_u
     double h_u, mu_u;
     double b_u[3];
+    // LCE shells: the fused step keeps each tile entry's dialled metric
+    // lambda^2 t t + lambda^-2nu r r (before any ansatz blend) in the entry's
+    // unused bbar-info record slots, and reuses it while the dial factor
+    // stays lce_f and the tile records are generation lce_gen (-1: none).
+    double lce_f;
+    long long lce_gen;
 };
 
 // Dial factor a step uses (main.cpp:286-288, 483-485).

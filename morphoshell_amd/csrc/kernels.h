@@ -86,6 +86,8 @@ struct StepFlags {
     int check_equil;       // fused equilibrium reduction on this step
     int halt_on_eq;
     double tol_f, tol_v;
+    long long lce_gen;     // fused tile step: generation of the tile records whose
+                           // LCE metric cache it wrote; -1 otherwise
 };
 void launch_node_step(const Launch& L, const NodeData& nd, const Buffers& b, const Chunk* chunks, int nchunks,
                       ShellDev* shells, int num_shells, ClockDev* clock, bool mixed, const StepFlags& fl);
@@ -180,6 +182,7 @@ struct TileData {
     const float* rec32;            // REC_SLOTS * 64 floats per block
     const int* active;             // null: every tile; else the tiles to run (nactive)
     int nactive;
+    long long gen;                 // generation of the record copies (bumped at every rebuild)
 };
 // Record gather: component k < ncomp of stream `in` (SoA, [k*T + t]) into
 // slot `slot + k` of every entry j < Tp with tris[j] >= 0 (elem_bytes 4 or 8).

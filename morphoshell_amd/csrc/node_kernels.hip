@@ -154,6 +154,12 @@ __global__ __launch_bounds__(256) void bookkeeping_kernel(NodeKArgs k) {
         ShellDev* S = &k.shells[s];
         const int halted = S->halted;
         if (MODE == 0 && halted) continue;
+        if (MODE == 0) {
+            // the tile records of generation fl.lce_gen now hold the LCE
+            // metric at the dial factor this step used (-1: none do)
+            S->lce_f = S->auto_protocol ? S->f_targets : shell_dial_factor(*S);
+            S->lce_gen = k.fl.lce_gen;
+        }
         const bool proto_check = MODE == 0 && S->auto_protocol && S->check_due;
         if (CHECK || proto_check) {
             double fm = -1.0, vm = -1.0;

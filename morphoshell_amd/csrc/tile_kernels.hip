@@ -130,7 +130,10 @@ void tile_step_kernel(TileKArgs k) {
         const bool do_check = CHECK || uni(sh.check_due);
         const int par = uni(sh.par);
         const double* X = par ? k.b.x[1] : k.b.x[0];
-        const TriParams tp = tri_params<UNI>(sh);
+        TriParams tp = tri_params<UNI>(sh);
+        // LCE shells: the records hold the dialled metric of this dial factor
+        // when an earlier fused step on these records used the same factor
+        if (LCE) tp.lce_cached = (tp.protocol == 3 && sh.lce_gen == k.tl.gen && sh.lce_f == tp.f) ? 1 : 0;
         const int nn = ch.y - ch.x;
         const int h0 = kload(k.tl.halo_off, ci), h1 = kload(k.tl.halo_off, ci + 1);
         const int j0 = kload(k.tl.tri_off, ci), j1 = kload(k.tl.tri_end, ci);
@@ -205,10 +208,17 @@ void tile_step_kernel(TileKArgs k) {
         int jn = j0 + (int)threadIdx.x;
         RecF64 nrec;
         unsigned nlw[3], ndw[3];
+        // entry j's bbar-info record slots, where an LCE shell's dialled
+        // metric is kept (null when the records already hold it)
+        auto keep = [&](int j) -> double* {
+            if (!LCE || tp.lce_cached) return nullptr;
+            return const_cast<double*>(k.tl.rec64) + (size_t)uni(j >> 6) * (REC_SLOTS * 64) + RS_BI * 64 + (j & 63);
+        };
         auto fetch = [&](int j) {
             words(k.tl.lpid, j, nlw);
             words(k.tl.dst, j, ndw);
             load_rec_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, nrec);
+            nrec.dr.wb = keep(j);
         };
         if (!MIXED && jn < j1) fetch(jn);
         // FP64: the node state of the thread's nodes (velocity, mass, clamp
@@ -266,7 +276,10 @@ void tile_step_kernel(TileKArgs k) {
                     unpack(lw, lp);
                     const TilePos pos{lpos, lp};
                     float out[18];
-                    tri_forces_mixed<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, pos, out);
+                    RecMixed rec;
+                    load_rec_mixed<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, rec);
+                    rec.dr.wb = keep(j);
+                    tri_forces_mixed_rec<LCE>(rec, tp, pos, out);
                     // the mixed kernel, at its 128-register limit, loads the
                     // LDS positions at the store (loaded with the ids: 5% slower)
                     words(k.tl.dst, j, dw);

@@ -78,6 +78,7 @@ struct TriParams {
     double nu, gamma, f;  // Poisson ratio, stiffening scale, dial factor
     int protocol, from_ansatz, has_a0, has_b0;
     double h, mu, b[3];   // uniform h, mu, bbar_info (UNI kernels only)
+    int lce_cached;       // fused step, LCE shell: the records' metric cache holds f's metric
 };
 
 // Factor the programmed metric is evaluated at: under the device-run protocol
@@ -100,6 +101,7 @@ __device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
     p.from_ansatz = uni(s.from_ansatz);
     p.has_a0 = uni(s.has_a0);
     p.has_b0 = uni(s.has_b0);
+    p.lce_cached = 0;
     if (UNI) {
         p.h = uni(s.h_u);
         p.mu = uni(s.mu_u);
@@ -116,6 +118,9 @@ __device__ __forceinline__ TriParams tri_params(const ShellDev& s) {
 // costs one memory round trip.
 struct DialRec {
     double ai[5], bi[3], A0[3], B0[3];
+    // fused step, LCE shell without a valid cache: where to keep the dialled
+    // metric (3 values at a stride of 64, the entry's bbar-info record slots)
+    double* wb = nullptr;
 };
 
 // CS: component stride of the record streams when known at compile time (the
@@ -123,11 +128,19 @@ struct DialRec {
 template <bool LCE, bool UNI, int CS = 0, bool NT = false>
 __device__ __forceinline__ void load_dial(const TriData& td, const TriParams& s, unsigned t, DialRec& d) {
     const size_t T = CS ? (size_t)CS : (size_t)td.T;
+    if (!(LCE && s.protocol == 3 && s.lce_cached)) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) d.ai[k] = ld_stream<NT>(td.ainfo + k * T, t);
+        for (int k = 0; k < 3; ++k) d.ai[k] = ld_stream<NT>(td.ainfo + k * T, t);
+    }
     if (LCE && s.protocol == 3) {
-        d.ai[3] = ld_stream<NT>(td.ainfo + 3 * T, t);
-        d.ai[4] = ld_stream<NT>(td.ainfo + 4 * T, t);
+        if (s.lce_cached) {
+            // the metric this dial factor gives, kept by an earlier step
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d.ai[k] = ld_stream<NT>(td.binfo + k * T, t);
+        } else {
+            d.ai[3] = ld_stream<NT>(td.ainfo + 3 * T, t);
+            d.ai[4] = ld_stream<NT>(td.ainfo + 4 * T, t);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) d.bi[k] = UNI ? s.b[k] : ld_stream<NT>(td.binfo + k * T, t);
@@ -156,13 +169,22 @@ __device__ __forceinline__ void dial_targets(const DialRec& d, const TriParams& 
     const double f = s.f;
     const double fin = s.from_ansatz ? 1.0 : f;
     if (LCE && s.protocol == 3) {
-        const double c2 = d.ai[0], cs = d.ai[1], s2 = d.ai[2];
-        const double lamf = d.ai[3], m2nu = d.ai[4];
-        const double lam = (1.0 - fin) * 1.0 + fin * lamf;
-        const double l2 = lam * lam, lp = pow(lam, m2nu);
-        ab[0] = l2 * c2 + lp * s2;
-        ab[1] = (l2 - lp) * cs;
-        ab[2] = lp * c2 + l2 * s2;
+        if (s.lce_cached) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ab[k] = d.ai[k];
+        } else {
+            const double c2 = d.ai[0], cs = d.ai[1], s2 = d.ai[2];
+            const double lamf = d.ai[3], m2nu = d.ai[4];
+            const double lam = (1.0 - fin) * 1.0 + fin * lamf;
+            const double l2 = lam * lam, lp = pow(lam, m2nu);
+            ab[0] = l2 * c2 + lp * s2;
+            ab[1] = (l2 - lp) * cs;
+            ab[2] = lp * c2 + l2 * s2;
+            if (d.wb) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) d.wb[64 * k] = ab[k];
+            }
+        }
         bb[0] = bb[1] = bb[2] = 0.0;
     } else {
 #pragma unroll

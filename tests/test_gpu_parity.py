@@ -730,6 +730,57 @@ def test_lce_shell_fused_equals_two_launch_while_dialling(precision):
         assert ra[0]["max_force_ratio"] == rb[0]["max_force_ratio"]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lce_metric_cache_equals_recomputing(precision):
+    """The fused step keeps an LCE shell's dialled metric in its tile records
+    and reuses it while the dial factor stays put.  Against the two-launch
+    form, which evaluates the opto-thermal law at every step, the states
+    stay bit-identical: while dialling (the cache is rewritten every step),
+    at the dial's end (f = 1 for many steps: reused), after the host moves
+    the dial clock, after the tiles are rebuilt (a new record generation),
+    across a precision switch and under the device-run protocol.  A blend
+    shell in the same context keeps its bbar-info record slots."""
+    shells = [_shell(0.06, seed=8, protocol=3), _shell(0.08, seed=9, kappa=0.6)]
+    states = [deformed_state(s, seed=91 + i) for i, s in enumerate(shells)]
+    other = "mixed" if precision == "fp64" else "fp64"
+    with _ctx(shells, precision) as a, _ctx(shells, precision) as b:
+        a.set_tuning(node_block=128, fused=1)
+        b.set_tuning(node_block=128, fused=0)
+
+        def same():
+            for i in range(len(shells)):
+                assert np.array_equal(a.get_state(i)[0], b.get_state(i)[0]), i
+                assert np.array_equal(a.get_state(i)[1], b.get_state(i)[1]), i
+
+        for c in (a, b):
+            for i, st in enumerate(states):
+                c.set_state(i, *st)
+                c.set_dial_clock(i, 0.0, 1.0, 40, 0, True)
+        for c in (a, b):
+            c.step(90, equil_every=25)  # 40 dialled steps, then 50 at f = 1
+        same()
+        for c in (a, b):
+            c.set_dial_clock(0, 0.5, 0.0, 40, 0, False)  # held at f = 0.5
+            c.step(30)
+        same()
+        a.set_tuning(node_block=64, fused=1)  # new tiles, new record generation
+        for c in (a, b):
+            c.step(30)
+        same()
+        for c in (a, b):
+            c.set_precision(other)
+            c.step(20)
+        same()
+        for c in (a, b):
+            c.set_precision(precision)
+            for i in range(len(shells)):
+                c.relax_setup(i, steps_per_dial_phase=30, dial_spacing=0.5, dial_start=0.0, equil_interval=10,
+                              tol_f=1e-30, tol_v=1e-30)
+            c.relax_run(150, poll_every=50)
+        same()
+        assert a.dial_factor(0) == b.dial_factor(0)
+
+
 @pytest.mark.parametrize("precision,fused", [("fp64", 0), ("mixed", 0), ("fp64", 1), ("mixed", 1)])
 def test_halted_shells_drop_out_of_the_step_launches(precision, fused):
     """Once shells halt, the step launches cover the running shells' chunks
