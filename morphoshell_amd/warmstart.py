@@ -326,11 +326,21 @@ def relax_batch(shells, tol: float, max_steps: int, plans=None, dial_steps: int 
                 a, b, _ = ctx.calc_a_comps_and_b_comps_and_normals(i)
                 ctx.set_zero_dial_geometry(i, a.reshape(-1), b.reshape(-1))
             starts.append((f0, K))
-        ctx.set_tuning(node_block=128, fused=1)  # small shells: more, smaller tiles
+        ctx.set_tuning(node_block=128, fused=1, tile_nodes=-2)  # small shells: small tiles, re-cut as they halt
         for i, (f0, K) in enumerate(starts):
             ctx.relax_setup(i, steps_per_dial_phase=K, dial_spacing=1.0, dial_start=f0,
                             equil_interval=equil_interval, tol_f=tol, tol_v=tol)
-        reps = ctx.relax_run(max_steps, poll_every=poll)
+        # in parts of at most 200 000 steps, with a progress line after each
+        done, reps = 0, []
+        while done < max_steps:
+            n = min(200_000, max_steps - done)
+            reps = ctx.relax_run(n, poll_every=poll)
+            done += n
+            running = sum(1 for r in reps if not r["halted"])
+            log(f"relax: {done} steps, {len(shells) - running} of {len(shells)} shells halted, "
+                f"{time.perf_counter() - t0:.1f}s")
+            if running == 0:
+                break
         ends = [ctx.get_state(i)[0] for i in range(len(shells))]
     return reps, ends, time.perf_counter() - t0
 
@@ -468,15 +478,19 @@ def warm_start_plans(net: WarmStartNet, shells, dial_steps: int = 2000, start: s
 
 
 def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: int = 2000, tol: float = 1e-5,
-             max_steps: int = 600000, family: str = "defect", start: str = "ansatz") -> dict:
+             max_steps: int = 600000, family: str = "defect", start: str = "ansatz", tmin: int = 1000,
+             tmax: int = 6000) -> dict:
     """Steps to equilibrium from the learned warm start (started at dial
     factor 1, or dialled from its own geometry: start, see warm_start_plans;
     with the fallback) against the flat start with noise dialled as usual, on
-    unseen seeds; final energies of both at dial factor 1."""
-    shells = training_family(num, seed, family=family)
+    unseen seeds of the family with triangle counts log-uniform in [tmin,
+    tmax]; final energies of both at dial factor 1."""
+    shells = training_family(num, seed, tmin=tmin, tmax=tmax, family=family)
+    log(f"evaluation: {num} shells of {tmin}-{tmax} triangles set up")
     t0 = time.perf_counter()
     plans, used = warm_start_plans(net, shells, dial_steps, start=start)
     t_pred = time.perf_counter() - t0
+    log(f"predictions: {sum(u is not None for u in used)} of {num} kept, {t_pred:.1f}s")
     base, base_end, t_base = relax_batch(shells, tol, max_steps, dial_steps=dial_steps)
     warm, warm_end, t_warm = relax_batch(shells, tol, max_steps, plans=plans, dial_steps=dial_steps)
     e_base = full_dial_energies(shells, base_end)
@@ -503,6 +517,7 @@ def evaluate(net: WarmStartNet, num: int = 64, seed: int = 10_000, dial_steps: i
         "final_energy_lower_or_equal_fraction": f(np.mean(de <= 1e-6)) if len(de) else None,
         "tolerance": tol,
         "start": start,
+        "triangles": [int(tmin), int(tmax)],
     }
 
 
@@ -531,10 +546,13 @@ def main(argv=None) -> int:
     ap.add_argument("--params", default="", help="trained parameters to evaluate (skips generation and training)")
     ap.add_argument("--start", default="ansatz", choices=("direct", "ansatz"),
                     help="warm start at dial factor 1, or dialled from the prediction's own geometry")
+    ap.add_argument("--eval-tmin", type=int, default=1000, help="evaluation shells: fewest triangles")
+    ap.add_argument("--eval-tmax", type=int, default=6000, help="evaluation shells: most triangles")
     args = ap.parse_args(argv)
     if args.params:
-        net = WarmStartNet.load(args.params)
-        ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start)
+        net = WarmStartNet.load(PRETRAINED_DEFECT if args.params == "pretrained" else args.params)
+        ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start,
+                      tmin=args.eval_tmin, tmax=args.eval_tmax)
         print(json.dumps({"params": args.params, "eval": ev}), flush=True)
         return 0
     t0 = time.time()
@@ -552,7 +570,8 @@ def main(argv=None) -> int:
     hist = train(net, data, epochs=epochs, lr=args.lr)
     t_train = time.time() - t0
     net.save(args.out)
-    ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start)
+    ev = evaluate(net, args.num_eval, max_steps=args.max_steps, family=args.family, start=args.start,
+                  tmin=args.eval_tmin, tmax=args.eval_tmax)
     print(json.dumps({"equilibria": len(data), "generate_s": t_gen, "train_s": t_train,
                       "loss_first": hist[0], "loss_last": hist[-1], "eval": ev}), flush=True)
     return 0
