@@ -83,6 +83,7 @@ struct ms_ctx {
     int64_t tile_rec_bytes = 0;      // record copies
     int acols = 3;
     bool uni_data = true;            // see uni_data()
+    bool area_ref = false;           // see TileData::area_ref
 
     // graphs
     hipGraphExec_t graph_exec = nullptr;
@@ -463,6 +464,7 @@ int build_tiles(ms_ctx* c) {
     td.max_passes = max_passes;
     td.max_nodes = max_nodes;
     td.max_entries = max_entries;
+    td.area_ref = c->area_ref ? 1 : 0;
     const int lds64 = tile_lds_bytes(td, false), lds32 = tile_lds_bytes(td, true);
     if (lds64 + 1024 > 160 * 1024)
         return fail(c, MS_ERR_ARG, fmt("fused step: a tile needs %d bytes of LDS (limit 160 KiB)", lds64));
@@ -914,6 +916,21 @@ extern "C" int ms_create(const ms_shell_desc* shells, int32_t num_shells, int32_
         }
         tb += d.num_tris;
         nb += d.num_nodes;
+    }
+    // The FP64 tile step can form a triangle's reference area from the
+    // reference x, y it stages for the patch fit: 0.5 |x1 y2 - x2 y1| over the
+    // edge vectors from patch node 0 to patch nodes 1 and 2 (the triangle's
+    // vertices), each product rounded.  It does so when that reproduces every
+    // given area bit for bit, as it does for ms_mesh_info's areas of a planar
+    // reference state (whose normal has no x, y component).
+    c->area_ref = true;
+    for (int t = 0; t < T && c->area_ref; ++t) {
+        const int p0 = pid[(size_t)t], p1 = pid[(size_t)T + t], p2 = pid[2 * (size_t)T + t];
+        const double x1 = rxy[p1] - rxy[p0], y1 = rxy[(size_t)N + p1] - rxy[(size_t)N + p0];
+        const double x2 = rxy[p2] - rxy[p0], y2 = rxy[(size_t)N + p2] - rxy[(size_t)N + p0];
+        volatile double q1 = x1 * y2, q2 = x2 * y1;  // rounded apart, as on the device
+        const double a = 0.5 * std::fabs(q1 - q2);
+        c->area_ref = std::memcmp(&a, &ar[t], sizeof(double)) == 0;
     }
     // patch-incidence CSR: entries t*6+slot grouped by node, ascending
     std::vector<long long> off((size_t)N + 1, 0);

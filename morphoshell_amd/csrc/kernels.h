@@ -183,6 +183,9 @@ struct TileData {
     const int* active;             // null: every tile; else the tiles to run (nactive)
     int nactive;
     long long gen;                 // generation of the record copies (bumped at every rebuild)
+    int area_ref;                  // 1: the FP64 step forms each reference area from the staged
+                                   // reference positions (equal to the area stream, checked at
+                                   // set-up) and does not read the area slot
 };
 // Record gather: component k < ncomp of stream `in` (SoA, [k*T + t]) into
 // slot `slot + k` of every entry j < Tp with tris[j] >= 0 (elem_bytes 4 or 8).

@@ -217,7 +217,7 @@ void tile_step_kernel(TileKArgs k) {
         auto fetch = [&](int j) {
             words(k.tl.lpid, j, nlw);
             words(k.tl.dst, j, ndw);
-            load_rec_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, nrec);
+            load_rec_fp64<LCE, UNI, 64>(view(j), tp, (unsigned)j & 63u, nrec, !k.tl.area_ref);
             nrec.dr.wb = keep(j);
         };
         if (!MIXED && jn < j1) fetch(jn);
@@ -290,7 +290,7 @@ void tile_step_kernel(TileKArgs k) {
                     unsigned lp[6], dw[3] = {ndw[0], ndw[1], ndw[2]};
                     unpack(nlw, lp);
                     double out[18];
-                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, TileRef{lref, lp}, out);
+                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, TileRef{lref, lp}, out, k.tl.area_ref != 0);
                     store(dw, reinterpret_cast<const CT*>(out));
                     if (jn + B < j1) fetch(jn + B);
                 }

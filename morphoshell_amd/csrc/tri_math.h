@@ -315,25 +315,38 @@ struct RecF64 {
     DialRec dr;
 };
 
+// load_area = false: the area is formed from the reference positions
+// (tri_forces_fp64_rec's area_ref) and not read.
 template <bool LCE, bool UNI, int CS = 0>
-__device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams& sp, unsigned t, RecF64& r) {
+__device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams& sp, unsigned t, RecF64& r,
+                                              bool load_area = true) {
     const size_t T = CS ? (size_t)CS : (size_t)td.T;
 #pragma unroll
     for (int k = 0; k < 9; ++k) r.hc[k] = ld_stream<CS != 0>(td.H + (9 + k) * T, t);
     r.h = UNI ? sp.h : ld_stream<CS != 0>(td.h, t);
     r.mu = UNI ? sp.mu : ld_stream<CS != 0>(td.mu, t);
-    r.A = ld_stream<CS != 0>(td.area, t);
+    r.A = load_area ? ld_stream<CS != 0>(td.area, t) : 0.0;
     load_dial<LCE, UNI, CS, CS != 0>(td, sp, t, r.dr);  // tile records (CS = 64): non-temporal
 }
 
+// 0.5 |x1 y2 - x2 y1|, each product rounded (no fused multiply-add): the
+// area of a triangle in the plane from its edge vectors (x1, y1), (x2, y2).
+__device__ __forceinline__ double planar_area(double x1, double y1, double x2, double y2) {
+#pragma clang fp contract(off)
+    return 0.5 * fabs(x1 * y2 - x2 * y1);
+}
+
 // FP64 force contributions (out[3*slot + c]) of the triangle whose record is
-// r; pos fetches its six patch nodes, ref their reference x, y.
+// r; pos fetches its six patch nodes, ref their reference x, y.  area_ref:
+// the reference area is 0.5 |x1 y2 - x2 y1| of the reference edge vectors to
+// patch nodes 1, 2 (each product rounded; the host checked that this equals
+// the area stream), else r.A.
 template <bool LCE, class Pos, class Ref>
 __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriParams& sp, const Pos& pos,
-                                                    const Ref& ref, double out[18]) {
+                                                    const Ref& ref, double out[18], bool area_ref = false) {
 #pragma clang fp contract(on)
     const double* hc = r.hc;
-    const double h = r.h, mu = r.mu, A = r.A;
+    const double h = r.h, mu = r.mu;
     const double nu = sp.nu;
     // reference edge vectors to patch node 0, and the first-derivative
     // weights gc of P_1 - P_0, P_2 - P_0: (G[0][1], G[0][2], G[1][1], G[1][2])
@@ -350,6 +363,7 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
         }
     }
     const double idet = 1.0 / (xi[1] * eta[2] - xi[2] * eta[1]);
+    const double A = area_ref ? planar_area(xi[1], eta[1], xi[2], eta[2]) : r.A;
     const double gc[4] = {eta[2] * idet, -eta[1] * idet, -xi[2] * idet, xi[1] * idet};
     // the vertex rows of H from the moment identities
     double hv[3][3];  // hv[v][k], v = 0, 1, 2

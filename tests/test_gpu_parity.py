@@ -345,6 +345,37 @@ def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid, tile_
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
 
 
+def test_fused_step_forms_areas_only_where_they_match(sh):
+    """The FP64 fused step forms each reference area as 0.5 |x1 y2 - x2 y1|
+    from the reference positions of patch nodes 0-2 when that equals every
+    given area bit for bit (so the fused runs of the other tests, on planar
+    reference states, take that path), and reads the area stream otherwise:
+    with one area scaled by 1 + 1e-9 the fused step still equals the
+    two-launch form, which reads the stream, and differs from the run on the
+    unperturbed areas."""
+    import copy
+
+    p = np.asarray(sh.patch_ids)
+    X, Y = sh.ref_pos[: sh.N], sh.ref_pos[sh.N: 2 * sh.N]
+    x1, y1 = X[p[:, 1]] - X[p[:, 0]], Y[p[:, 1]] - Y[p[:, 0]]
+    x2, y2 = X[p[:, 2]] - X[p[:, 0]], Y[p[:, 2]] - Y[p[:, 0]]
+    assert np.array_equal(0.5 * np.abs(x1 * y2 - x2 * y1), sh.areas)
+    s2 = copy.copy(sh)
+    s2.areas = sh.areas.copy()
+    s2.areas[sh.T // 3] *= 1.0 + 1e-9
+    x, v = deformed_state(sh, seed=41)
+    with _ctx(s2) as a, _ctx(s2) as b, _ctx(sh) as c:
+        for ctx in (a, b, c):
+            ctx.set_state(0, x, v)
+        b.set_tuning(node_block=256, fused=1)
+        c.set_tuning(node_block=256, fused=1)
+        for ctx in (a, b, c):
+            ctx.step(20)
+        xa, xb, xc = (ctx.get_state(0)[0] for ctx in (a, b, c))
+    assert np.array_equal(xa, xb)
+    assert not np.array_equal(xb, xc)
+
+
 @pytest.mark.parametrize("tile_chunks", [1, 2])
 def test_fused_tile_step_batch_matches_oracle(tile_chunks):
     shells = [_shell(0.08, seed=1), _shell(0.05, seed=2, kappa=-1.0, clamp_every=7), _shell(0.11, seed=3, h=0.02)]
