@@ -290,7 +290,7 @@ void tile_step_kernel(TileKArgs k) {
                     unsigned lp[6], dw[3] = {ndw[0], ndw[1], ndw[2]};
                     unpack(nlw, lp);
                     double out[18];
-                    tri_forces_fp64_rec<LCE>(nrec, tp, TilePos{lpos, lp}, TileRef{lref, lp}, out, k.tl.area_ref != 0);
+                    tri_forces_fp64_rec<LCE, UNI>(nrec, tp, TilePos{lpos, lp}, TileRef{lref, lp}, out, k.tl.area_ref != 0);
                     store(dw, reinterpret_cast<const CT*>(out));
                     if (jn + B < j1) fetch(jn + B);
                 }

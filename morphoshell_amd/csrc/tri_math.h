@@ -329,6 +329,14 @@ __device__ __forceinline__ void load_rec_fp64(const TriData& td, const TriParams
     load_dial<LCE, UNI, CS, CS != 0>(td, sp, t, r.dr);  // tile records (CS = 64): non-temporal
 }
 
+// 1/x to about an ulp for normal, finite x: the hardware reciprocal refined
+// by two Newton steps (no final rounding correction).
+__device__ __forceinline__ double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
 // 0.5 |x1 y2 - x2 y1|, each product rounded (no fused multiply-add): the
 // area of a triangle in the plane from its edge vectors (x1, y1), (x2, y2).
 __device__ __forceinline__ double planar_area(double x1, double y1, double x2, double y2) {
@@ -341,12 +349,14 @@ __device__ __forceinline__ double planar_area(double x1, double y1, double x2, d
 // the reference area is 0.5 |x1 y2 - x2 y1| of the reference edge vectors to
 // patch nodes 1, 2 (each product rounded; the host checked that this equals
 // the area stream), else r.A.
-template <bool LCE, class Pos, class Ref>
+// UNI: h and mu are the shell's (sp), so the factors built from them alone
+// are computed once per tile.
+template <bool LCE, bool UNI, class Pos, class Ref>
 __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriParams& sp, const Pos& pos,
                                                     const Ref& ref, double out[18], bool area_ref = false) {
 #pragma clang fp contract(on)
     const double* hc = r.hc;
-    const double h = r.h, mu = r.mu;
+    const double h = UNI ? sp.h : r.h, mu = UNI ? sp.mu : r.mu;
     const double nu = sp.nu;
     // reference edge vectors to patch node 0, and the first-derivative
     // weights gc of P_1 - P_0, P_2 - P_0: (G[0][1], G[0][2], G[1][1], G[1][2])
@@ -362,7 +372,7 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
             eta[s] = Y - Y0;
         }
     }
-    const double idet = 1.0 / (xi[1] * eta[2] - xi[2] * eta[1]);
+    const double idet = fast_rcp(xi[1] * eta[2] - xi[2] * eta[1]);
     const double A = area_ref ? planar_area(xi[1], eta[1], xi[2], eta[2]) : r.A;
     const double gc[4] = {eta[2] * idet, -eta[1] * idet, -xi[2] * idet, xi[1] * idet};
     // the vertex rows of H from the moment identities
@@ -416,8 +426,9 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     const double m0 = Rx[1] * Ry[2] - Rx[2] * Ry[1];
     const double m1 = Rx[2] * Ry[0] - Rx[0] * Ry[2];
     const double m2 = Rx[0] * Ry[1] - Rx[1] * Ry[0];
-    const double mn = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
-    const double imn = 1.0 / mn;
+    // reciprocal square roots (a few ulp) and reciprocals of per-shell values
+    // in place of per-triangle FP64 divisions and square roots
+    const double imn = rsqrt(m0 * m0 + m1 * m1 + m2 * m2);
     const double n[3] = {m0 * imn, m1 * imn, m2 * imn};
     double b[3];
 #pragma unroll
@@ -426,16 +437,17 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     dial_targets<LCE>(r.dr, sp, ab, bb);
     const double P[3] = {ab[2], -ab[1], ab[0]};
     const double det = ab[0] * ab[2] - ab[1] * ab[1];
-    const double idet32 = 1.0 / (det * sqrt(det));
-    const double cs = h * mu / (1.0 - nu) * idet32;
-    const double cb = mu * h * h * h / (12.0 * (1.0 - nu)) * idet32;
+    const double rdet = rsqrt(det);
+    const double idet32 = rdet * rdet * rdet;  // det^(-3/2)
+    const double i1mnu = 1.0 / (1.0 - nu);
+    const double cs = h * mu * i1mnu * idet32;
+    const double cb = mu * h * h * h * (i1mnu * (1.0 / 12.0)) * idet32;
     const double Da[3] = {a0 - ab[0], a1 - ab[1], a2 - ab[2]};
     const double Db[3] = {b[0] - bb[0], b[1] - bb[1], b[2] - bb[2]};
-    const double scale = sp.gamma * mu * h;
     double ga[3], gbv[3];
     phi_grad(nu, P, Da, ga);
     const double Wb0 = cb * phi_grad(nu, P, Db, gbv);
-    const double stiff = 1.0 + 2.0 * Wb0 / scale;  // calc_deformation_forces.cpp:141
+    const double stiff = 1.0 + Wb0 * (2.0 / (sp.gamma * mu * h));  // calc_deformation_forces.cpp:141
     const double Wa[3] = {cs * ga[0], cs * ga[1], cs * ga[2]};
     const double cbs = cb * stiff;
     const double Wb[3] = {cbs * gbv[0], cbs * gbv[1], cbs * gbv[2]};
@@ -477,7 +489,7 @@ __device__ __forceinline__ void tri_forces_fp64(const TriData& td, const TriPara
                                                 const Ref& ref, double out[18]) {
     RecF64 r;
     load_rec_fp64<LCE, UNI, CS>(td, sp, t, r);
-    tri_forces_fp64_rec<LCE>(r, sp, pos, ref, out);
+    tri_forces_fp64_rec<LCE, UNI>(r, sp, pos, ref, out);
 }
 
 // The mixed-mode record of triangle t: 19 of the 24 FP32 patch coefficients
