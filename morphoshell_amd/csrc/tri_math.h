@@ -375,48 +375,42 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     const double idet = fast_rcp(xi[1] * eta[2] - xi[2] * eta[1]);
     const double A = area_ref ? planar_area(xi[1], eta[1], xi[2], eta[2]) : r.A;
     const double gc[4] = {eta[2] * idet, -eta[1] * idet, -xi[2] * idet, xi[1] * idet};
-    // the vertex rows of H from the moment identities
-    double hv[3][3];  // hv[v][k], v = 0, 1, 2
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk) {
-        const double s0 = (hc[kk] + hc[3 + kk]) + hc[6 + kk];
-        const double t1 = hc[kk] * xi[3] + hc[3 + kk] * xi[4] + hc[6 + kk] * xi[5];
-        const double t2 = hc[kk] * eta[3] + hc[3 + kk] * eta[4] + hc[6 + kk] * eta[5];
-        hv[1][kk] = -(gc[0] * t1 + gc[2] * t2);
-        hv[2][kk] = -(gc[1] * t1 + gc[3] * t2);
-        hv[0][kk] = -((hv[1][kk] + hv[2][kk]) + s0);
-    }
     double x0, y0, z0;
     pos(0, x0, y0, z0);
-    // first and second derivatives from the edge vectors to patch node 0
+    // First derivatives from the edge vectors e_1, e_2 to patch node 0.  The
+    // vertex rows of H follow from the moment identities (its columns
+    // annihilate 1, X and Y): (H_1, H_2) = -E^-T sum_n H_n (xi_n, eta_n) over
+    // the non-vertex nodes n = 3..5, H_0 = -(H_1 + H_2 + sum_n H_n).  So the
+    // second derivatives are R2 = sum_n H_n d_n with d_n = e_n - xi_n Rx -
+    // eta_n Ry, node n's offset from the tangent-plane prediction, and the
+    // vertex rows are never formed.
     double Rx[3], Ry[3], R2[3][3];  // R2[k][c]: k = xx, xy, yy
+    {
+        double p1[3], p2[3];
+        pos(1, p1[0], p1[1], p1[2]);
+        pos(2, p2[0], p2[1], p2[2]);
 #pragma unroll
-    for (int s = 1; s < 6; ++s) {
-        double px, py, pz;
-        pos(s, px, py, pz);
-        const double e[3] = {px - x0, py - y0, pz - z0};
-        if (s == 1) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                Rx[c] = gc[0] * e[c];
-                Ry[c] = gc[2] * e[c];
-            }
-        } else if (s == 2) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                Rx[c] += gc[1] * e[c];
-                Ry[c] += gc[3] * e[c];
-            }
+        for (int c = 0; c < 3; ++c) {
+            const double e1 = p1[c] - (c == 0 ? x0 : c == 1 ? y0 : z0);
+            const double e2 = p2[c] - (c == 0 ? x0 : c == 1 ? y0 : z0);
+            Rx[c] = gc[0] * e1 + gc[1] * e2;
+            Ry[c] = gc[2] * e1 + gc[3] * e2;
         }
+    }
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-            const double w = s < 3 ? hv[s][kk] : hc[3 * (s - 3) + kk];
+    for (int s = 3; s < 6; ++s) {
+        double p[3];
+        pos(s, p[0], p[1], p[2]);
+        const double* hs = hc + 3 * (s - 3);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (s == 1)
-                    R2[kk][c] = w * e[c];
+        for (int c = 0; c < 3; ++c) {
+            const double d = (p[c] - (c == 0 ? x0 : c == 1 ? y0 : z0)) - xi[s] * Rx[c] - eta[s] * Ry[c];
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                if (s == 3)
+                    R2[kk][c] = hs[kk] * d;
                 else
-                    R2[kk][c] += w * e[c];
+                    R2[kk][c] += hs[kk] * d;
             }
         }
     }
@@ -469,13 +463,26 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     }
     // node 0's first-derivative weights: minus the sums of the others'
     const double g0[2] = {-(gc[0] + gc[1]), -(gc[2] + gc[3])};
+    // H_s . Wb per patch node: the non-vertex nodes' directly, the vertex
+    // nodes' through the moment identities above
+    double ws[6];
+#pragma unroll
+    for (int s = 3; s < 6; ++s) {
+        const double* hs = hc + 3 * (s - 3);
+        ws[s] = hs[0] * Wb[0] + hs[1] * Wb[1] + hs[2] * Wb[2];
+    }
+    {
+        const double T1 = ws[3] * xi[3] + ws[4] * xi[4] + ws[5] * xi[5];
+        const double T2 = ws[3] * eta[3] + ws[4] * eta[4] + ws[5] * eta[5];
+        ws[1] = -(gc[0] * T1 + gc[2] * T2);
+        ws[2] = -(gc[1] * T1 + gc[3] * T2);
+        ws[0] = -((ws[1] + ws[2]) + ((ws[3] + ws[4]) + ws[5]));
+    }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const double* hs = s < 3 ? hv[s] : hc + 3 * (s - 3);
-        const double ws = hs[0] * Wb[0] + hs[1] * Wb[1] + hs[2] * Wb[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double f = ws * An[c];
+            double f = ws[s] * An[c];
             if (s == 0) f -= g0[0] * Agx[c] + g0[1] * Agy[c];
             if (s == 1 || s == 2) f -= gc[s - 1] * Agx[c] + gc[s + 1] * Agy[c];
             out[3 * s + c] = f;
