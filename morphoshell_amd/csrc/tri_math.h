@@ -382,9 +382,10 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     // annihilate 1, X and Y): (H_1, H_2) = -E^-T sum_n H_n (xi_n, eta_n) over
     // the non-vertex nodes n = 3..5, H_0 = -(H_1 + H_2 + sum_n H_n).  So the
     // second derivatives are R2 = sum_n H_n d_n with d_n = e_n - xi_n Rx -
-    // eta_n Ry, node n's offset from the tangent-plane prediction, and the
-    // vertex rows are never formed.
-    double Rx[3], Ry[3], R2[3][3];  // R2[k][c]: k = xx, xy, yy
+    // eta_n Ry, node n's offset from the tangent-plane prediction.  R2 enters
+    // only through b = -n . R2 and Wb . R2, which are formed from the d_n
+    // directly; neither R2 nor the vertex rows are formed.
+    double Rx[3], Ry[3], d[3][3];  // d[n - 3][c]
     {
         double p1[3], p2[3];
         pos(1, p1[0], p1[1], p1[2]);
@@ -401,18 +402,9 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     for (int s = 3; s < 6; ++s) {
         double p[3];
         pos(s, p[0], p[1], p[2]);
-        const double* hs = hc + 3 * (s - 3);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double d = (p[c] - (c == 0 ? x0 : c == 1 ? y0 : z0)) - xi[s] * Rx[c] - eta[s] * Ry[c];
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {
-                if (s == 3)
-                    R2[kk][c] = hs[kk] * d;
-                else
-                    R2[kk][c] += hs[kk] * d;
-            }
-        }
+        for (int c = 0; c < 3; ++c)
+            d[s - 3][c] = (p[c] - (c == 0 ? x0 : c == 1 ? y0 : z0)) - xi[s] * Rx[c] - eta[s] * Ry[c];
     }
     const double a0 = Rx[0] * Rx[0] + Rx[1] * Rx[1] + Rx[2] * Rx[2];
     const double a1 = Rx[0] * Ry[0] + Rx[1] * Ry[1] + Rx[2] * Ry[2];
@@ -424,9 +416,12 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     // in place of per-triangle FP64 divisions and square roots
     const double imn = rsqrt(m0 * m0 + m1 * m1 + m2 * m2);
     const double n[3] = {m0 * imn, m1 * imn, m2 * imn};
-    double b[3];
+    // b = -n . R2 = -sum_n H_n (n . d_n)
+    double dn[3], b[3];
 #pragma unroll
-    for (int kk = 0; kk < 3; ++kk) b[kk] = -(n[0] * R2[kk][0] + n[1] * R2[kk][1] + n[2] * R2[kk][2]);
+    for (int j = 0; j < 3; ++j) dn[j] = n[0] * d[j][0] + n[1] * d[j][1] + n[2] * d[j][2];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) b[kk] = -(hc[kk] * dn[0] + hc[3 + kk] * dn[1] + hc[6 + kk] * dn[2]);
     double ab[3], bb[3];
     dial_targets<LCE>(r.dr, sp, ab, bb);
     const double P[3] = {ab[2], -ab[1], ab[0]};
@@ -442,27 +437,10 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     phi_grad(nu, P, Da, ga);
     const double Wb0 = cb * phi_grad(nu, P, Db, gbv);
     const double stiff = 1.0 + Wb0 * (2.0 / (sp.gamma * mu * h));  // calc_deformation_forces.cpp:141
-    const double Wa[3] = {cs * ga[0], cs * ga[1], cs * ga[2]};
-    const double cbs = cb * stiff;
-    const double Wb[3] = {cbs * gbv[0], cbs * gbv[1], cbs * gbv[2]};
-    double w[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) w[c] = -(Wb[0] * R2[0][c] + Wb[1] * R2[1][c] + Wb[2] * R2[2][c]);
-    const double nw = n[0] * w[0] + n[1] * w[1] + n[2] * w[2];
-    double u[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u[c] = (w[c] - nw * n[c]) * imn;
-    const double cx[3] = {Ry[1] * u[2] - Ry[2] * u[1], Ry[2] * u[0] - Ry[0] * u[2], Ry[0] * u[1] - Ry[1] * u[0]};
-    const double cy[3] = {u[1] * Rx[2] - u[2] * Rx[1], u[2] * Rx[0] - u[0] * Rx[2], u[0] * Rx[1] - u[1] * Rx[0]};
-    double Agx[3], Agy[3], An[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Agx[c] = A * (2.0 * Wa[0] * Rx[c] + Wa[1] * Ry[c] + cx[c]);
-        Agy[c] = A * (Wa[1] * Rx[c] + 2.0 * Wa[2] * Ry[c] + cy[c]);
-        An[c] = A * n[c];
-    }
-    // node 0's first-derivative weights: minus the sums of the others'
-    const double g0[2] = {-(gc[0] + gc[1]), -(gc[2] + gc[3])};
+    // the stresses times the reference area (every force term carries A once)
+    const double csA = cs * A, cbsA = cb * stiff * A;
+    const double Wa[3] = {csA * ga[0], csA * ga[1], csA * ga[2]};
+    const double Wb[3] = {cbsA * gbv[0], cbsA * gbv[1], cbsA * gbv[2]};
     // H_s . Wb per patch node: the non-vertex nodes' directly, the vertex
     // nodes' through the moment identities above
     double ws[6];
@@ -471,6 +449,24 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
         const double* hs = hc + 3 * (s - 3);
         ws[s] = hs[0] * Wb[0] + hs[1] * Wb[1] + hs[2] * Wb[2];
     }
+    // w = -Wb . R2 = -sum_n (H_n . Wb) d_n
+    double w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c] = -(ws[3] * d[0][c] + ws[4] * d[1][c] + ws[5] * d[2][c]);
+    const double nw = n[0] * w[0] + n[1] * w[1] + n[2] * w[2];
+    double u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = (w[c] - nw * n[c]) * imn;
+    const double cx[3] = {Ry[1] * u[2] - Ry[2] * u[1], Ry[2] * u[0] - Ry[0] * u[2], Ry[0] * u[1] - Ry[1] * u[0]};
+    const double cy[3] = {u[1] * Rx[2] - u[2] * Rx[1], u[2] * Rx[0] - u[0] * Rx[2], u[0] * Rx[1] - u[1] * Rx[0]};
+    double Agx[3], Agy[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Agx[c] = 2.0 * Wa[0] * Rx[c] + Wa[1] * Ry[c] + cx[c];
+        Agy[c] = Wa[1] * Rx[c] + 2.0 * Wa[2] * Ry[c] + cy[c];
+    }
+    // node 0's first-derivative weights: minus the sums of the others'
+    const double g0[2] = {-(gc[0] + gc[1]), -(gc[2] + gc[3])};
     {
         const double T1 = ws[3] * xi[3] + ws[4] * xi[4] + ws[5] * xi[5];
         const double T2 = ws[3] * eta[3] + ws[4] * eta[4] + ws[5] * eta[5];
@@ -482,7 +478,7 @@ __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriPa
     for (int s = 0; s < 6; ++s) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double f = ws[s] * An[c];
+            double f = ws[s] * n[c];
             if (s == 0) f -= g0[0] * Agx[c] + g0[1] * Agy[c];
             if (s == 1 || s == 2) f -= gc[s - 1] * Agx[c] + gc[s + 1] * Agy[c];
             out[3 * s + c] = f;
