@@ -350,7 +350,7 @@ __device__ __forceinline__ double planar_area(double x1, double y1, double x2, d
 // patch nodes 1, 2 (each product rounded; the host checked that this equals
 // the area stream), else r.A.
 // UNI: h and mu are the shell's (sp), so the factors built from them alone
-// are computed once per tile.
+// are loop invariants of the kernels' triangle loops.
 template <bool LCE, bool UNI, class Pos, class Ref>
 __device__ __forceinline__ void tri_forces_fp64_rec(const RecF64& r, const TriParams& sp, const Pos& pos,
                                                     const Ref& ref, double out[18], bool area_ref = false) {
