@@ -83,7 +83,7 @@ def signature(ctx: Context) -> str:
     S = len(ctx.shells)
     b = lambda v: int(round(math.log2(max(v, 1))))  # noqa: E731
     prec = "mixed" if ctx.precision == 1 else "fp64"
-    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-{_device()}-v5"
+    return f"T2^{b(T)}-N2^{b(N)}-S2^{b(S)}-{prec}-{_device()}-v6"
 
 
 def _load() -> dict:
