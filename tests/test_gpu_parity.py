@@ -345,6 +345,40 @@ def test_fused_tile_step_is_bit_identical(sh, precision, node_block, grid, tile_
         assert np.array_equal(a.get_state(0)[0], b.get_state(0)[0])
 
 
+@pytest.mark.parametrize("scale", [1e-5, 1e8])
+def test_forces_match_oracle_at_other_length_scales(scale):
+    """The FP64 mechanics use refined hardware reciprocals and reciprocal
+    square roots where the oracle divides: on a disc scaled by 1e-5 and 1e8
+    (thickness and target curvature with it), the forces still match the
+    oracle to 1e-10 of max |F|, and a fused step equals a two-launch step bit
+    for bit.  (The patch fit's rank rule, relative to the largest pivot of a
+    matrix whose rows scale as 1, L and L^2, as in the reference's
+    FullPivLU, refuses this mesh at scales of 1e-6 and 1e9.)"""
+    from morphoshell_amd.shell import Settings, prepare_shell
+
+    m = grf_disc(0.04, seed=3, h=0.01 * scale, kappa=0.5 / scale)
+    m.points = m.points * scale
+    s = prepare_shell(m, Settings(bend_stiffening_scale_factor=40.0, dial_protocol=1))
+    x, v = deformed_state(s, seed=21, amp=0.05 * scale)
+    x[2 * s.N:] = 0.05 * scale * np.sin(2.5 * x[:s.N] / scale) * np.cos(1.7 * x[s.N:2 * s.N] / scale)
+    v = v * scale
+    Fo = orc.System.from_shell(s).deformation_forces(x)
+    with _ctx(s) as a, _ctx(s) as b:
+        a.set_state(0, x, v)
+        a.calc_deformation_forces()
+        Fg = a.get_forces(0)
+        b.set_state(0, x, v)
+        b.set_tuning(node_block=128, fused=1)
+        a.step(1)
+        b.step(1)
+        xa, va = a.get_state(0)
+        xb, vb = b.get_state(0)
+    err = np.abs(Fg - Fo).max() / np.abs(Fo).max()
+    print(f"scale {scale:g}: max |F| {np.abs(Fo).max():.3e}, rel err {err:.3e}")
+    assert np.isfinite(Fg).all() and err <= 1e-10
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
 def test_fused_step_forms_areas_only_where_they_match(sh):
     """The FP64 fused step forms each reference area as 0.5 |x1 y2 - x2 y1|
     from the reference positions of patch nodes 0-2 when that equals every
