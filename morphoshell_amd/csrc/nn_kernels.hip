@@ -204,13 +204,19 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntiles = (a.M + 63) / 64;
     if ((int)blockIdx.x >= ntiles) return;  // block-uniform
-    S cur;
+    // A tile's gathered rows are loaded one tile ahead: the next tile's loads
+    // are in flight through this tile's GEMMs (its rows are disjoint from the
+    // rows this tile writes, and h is not written in the edge half).  The
+    // first tile's gather follows the weight staging: issued before it, the
+    // edge half gave wrong values (cause not found; the equality test of the
+    // fused inference against the training forward covers this form).
+    S cur, nxt;
     // weights, once per block: read k-major (coalesced over the output index), stored transposed
     for (int i = tid; i < K1 * W; i += 256) W1t[(i % W) * L1 + i / W] = f2bf(a.W1[i]);
     for (int i = tid; i < W * W; i += 256) W2t[(i % W) * L2 + i / W] = f2bf(a.W2[i]);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int r0 = tile * 64;
-        mlp_gather<W, EDGE>(a, r0, cur);
+        if (tile == (int)blockIdx.x) mlp_gather<W, EDGE>(a, r0, cur);
         // this tile's A rows into LDS as bfloat16, four values per 8-byte store
 #pragma unroll
         for (int j = 0; j < S::NQ; ++j) {
@@ -220,6 +226,7 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
             *reinterpret_cast<v4s*>(As + r * L1 + q * 4) = v4s{f2bf(v.x), f2bf(v.y), f2bf(v.z), f2bf(v.w)};
         }
         __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) mlp_gather<W, EDGE>(a, (tile + gridDim.x) * 64, nxt);
         v4f acc[W / 16];
         wave_gemm<W, K1>(As + wave * 16 * L1, L1, W1t, L1, acc);
         // bias, SiLU, rounded for the second GEMM (D: column lane & 15 of rows 4 (lane >> 4) + r)
@@ -254,6 +261,7 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(MlpLayerArgs a) {
             }
         }
         __syncthreads();  // the next tile restages As and Ts
+        cur = nxt;
     }
 }
 
